@@ -1,0 +1,96 @@
+"""Brute-force Fock-space Hamiltonian and exact FCI (independent check).
+
+Basis: K spatial sites, local state s = 2*n_alpha + n_beta (the order of the
+reference's local basis, qtensor_opers.py:104-106), site 0 most significant,
+Jordan-Wigner order over spin orbitals 2i (alpha), 2i+1 (beta).
+H = sum_pr h_pr E_pr + 1/2 sum_prqs (pr|qs) (E_pr E_qs - delta_qr E_ps)
+with spin-summed E_pr.  Test infrastructure only.
+"""
+import numpy as np
+import scipy.sparse as sp
+
+
+def _annihilators(K):
+    n = 2 * K
+    dim = 4 ** K
+    states = np.arange(dim)
+    # occupation bit of spin orbital p: site p//2, alpha (bit 1) or beta (bit 0)
+    occ = np.zeros((n, dim), dtype=np.int64)
+    for p in range(n):
+        site, spin = divmod(p, 2)
+        s = (states // 4 ** (K - 1 - site)) % 4
+        occ[p] = (s >> 1) & 1 if spin == 0 else s & 1
+    ops = []
+    for p in range(n):
+        site, spin = divmod(p, 2)
+        mask = occ[p] == 1
+        src = states[mask]
+        delta = (2 if spin == 0 else 1) * 4 ** (K - 1 - site)
+        dst = src - delta
+        sign = (-1.0) ** occ[:p, mask].sum(axis=0)
+        ops.append(sp.csr_matrix((sign, (dst, src)), shape=(dim, dim)))
+    return ops, occ
+
+
+def fock_hamiltonian(h, eri):
+    """Sparse H (4^K x 4^K) from spatial h[K,K] and chemists' eri[K,K,K,K]."""
+    K = h.shape[0]
+    a, occ = _annihilators(K)
+    E = [[None] * K for _ in range(K)]
+    for p in range(K):
+        for r in range(K):
+            E[p][r] = (a[2 * p].T @ a[2 * r] + a[2 * p + 1].T @ a[2 * r + 1]).tocsr()
+    dim = 4 ** K
+    H = sp.csr_matrix((dim, dim))
+    for p in range(K):
+        for r in range(K):
+            H = H + h[p, r] * E[p][r]
+    for p in range(K):
+        for r in range(K):
+            for q in range(K):
+                for s in range(K):
+                    v = eri[p, r, q, s]
+                    if v == 0.0:
+                        continue
+                    t = E[p][r] @ E[q][s]
+                    if q == r:
+                        t = t - E[p][s]
+                    H = H + 0.5 * v * t
+    return H.tocsr(), occ
+
+
+def sector_states(K, nelec, sz):
+    dim = 4 ** K
+    states = np.arange(dim)
+    na = np.zeros(dim, dtype=np.int64)
+    nb = np.zeros(dim, dtype=np.int64)
+    for site in range(K):
+        s = (states // 4 ** (K - 1 - site)) % 4
+        na += (s >> 1) & 1
+        nb += s & 1
+    return np.flatnonzero((na + nb == nelec) & (np.abs(0.5 * (na - nb) - sz) < 1e-12))
+
+
+def fci_energy(h, eri, nelec, sz=0.0, nroots=1):
+    """Lowest eigenvalues of H in the (N, Sz) sector."""
+    K = h.shape[0]
+    H, _ = fock_hamiltonian(h, eri)
+    idx = sector_states(K, nelec, sz)
+    Hs = H[idx][:, idx].toarray()
+    return np.linalg.eigvalsh(Hs)[:nroots]
+
+
+def mpo_dense(mpo_sites, K, nops):
+    """Sum over ops of the fully contracted MPO, as a 4^K x 4^K matrix."""
+    H = np.zeros((4 ** K, 4 ** K))
+    for p in range(nops):
+        t = mpo_sites[0][p].toDenseTensor()          # [1, wr, u, d]
+        t = t[0]                                     # [wr, u, d]
+        for i in range(1, K):
+            w = mpo_sites[i][p].toDenseTensor()      # [wl, wr, u, d]
+            t = np.tensordot(t, w, axes=([0], [0]))  # [U, D, wr, u, d]
+            t = t.transpose(2, 0, 3, 1, 4)           # [wr, U, u, D, d]
+            s = t.shape
+            t = t.reshape(s[0], s[1] * s[2], s[3] * s[4])
+        H += t[0]
+    return H
