@@ -1,0 +1,124 @@
+/* libzmpo -- C-ABI of the MI355X two-site sigma / renormalisation path.
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device addresses;
+ * `stream` is a hipStream_t passed as void* (NULL = default stream).  Every
+ * function returns ZMPO_OK (0) or a negative ZMPO_ERR_* code, and
+ * zmpo_last_error() returns the message of the calling thread's last error.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   zmpo_symAllowed      ctypes/qsym.c:5-22 (symAllowed), bound by ctypes in
+ *                        source/qtensor/qtensor_util.py:70-90 (blks_allowed1).
+ *   zmpo_symAllowedInt   integer-QN variant of the same rule (no counterpart;
+ *                        QNs as (N, 2Sz) int32, exact equality).
+ *   zmpo_blockgemm       the numeric loop of qtensor.tensordotCAL
+ *                        (source/qtensor/qtensor.py:642-659): every block
+ *                        pair of one or more tensordotSYM plans
+ *                        (qtensor.py:563-637) in one launch; used for the four
+ *                        sigma stages of HVecQt (source/mpo_dmrg_kernelQt.py:
+ *                        270-306), the three renormalisation stages of
+ *                        renormQt_H00 (kernelQt.py:674-683, 752-758) and
+ *                        genHopsQt (source/mpo_dmrg_initQt.py:146-152), and
+ *                        the final contraction of HDiagQt (kernelQt.py:116-119).
+ *   zmpo_dots, zmpo_lincomb, zmpo_precond
+ *                        the vector algebra of the Davidson solver
+ *                        (source/tools/pdvdson.py:131-283, 350-390), which the
+ *                        reference runs with numpy on rank 0.
+ *   zmpo_gather, zmpo_scatter, zmpo_convert
+ *                        qtensor.fromDense / toDenseTensor / prjmap index
+ *                        moves (qtensor.py:364-398, 442-453) and precision
+ *                        casts.
+ */
+#ifndef ZMPO_H
+#define ZMPO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZMPO_ABI_VERSION 1
+
+#define ZMPO_OK 0
+#define ZMPO_ERR_ARG (-1)
+#define ZMPO_ERR_HIP (-2)
+
+#define ZMPO_F64 0
+#define ZMPO_F32 1
+
+/* One output block of a grouped block GEMM.  Output element (m, n) of the
+ * block lives at c_off + (m / c_mlo) * c_smhi + (m % c_mlo) * c_smlo
+ * + n * c_sn.  tile_base is the exclusive prefix sum of the 64x64 tiles of
+ * the preceding tasks; tiles_n = ceil(N / 64). */
+typedef struct zmpo_task {
+  int64_t c_off;
+  int64_t tile_base;
+  int32_t M, N;
+  int32_t c_mlo, c_smhi, c_smlo, c_sn;
+  int32_t p_begin, p_end;
+  int32_t tiles_n, pad;
+} zmpo_task;
+
+/* One block pair: C_block += scale * A(M x K) * B(K x N), with
+ * A(m, k) = A[a_off + (m / a_mlo) * a_smhi + (m % a_mlo) * a_smlo + k * a_sk]
+ * B(k, n) = B[b_off + (n / b_nlo) * b_snhi + (n % b_nlo) * b_snlo + k * b_sk]. */
+typedef struct zmpo_pair {
+  int64_t a_off, b_off;
+  double scale;
+  int32_t K;
+  int32_t a_mlo, a_smhi, a_smlo, a_sk;
+  int32_t b_nlo, b_snhi, b_snlo, b_sk;
+  int32_t pad;
+} zmpo_pair;
+
+const char* zmpo_last_error(void);
+int zmpo_abi_version(void);
+int zmpo_struct_sizes(int* task_size, int* pair_size);
+
+/* Block selection rule with the reference's argument list (qsym.c:5-7):
+ * allowed[b] = (sum_{status=0} q - sum_{status=1} q has L1 norm < 1e-10)
+ * for every block b (C order over ndims).  QN arrays are (ndims[i], nqnum)
+ * row-major float64.  rank in [2, 8]; unlike the reference (qsym.c:436) the
+ * rank-8 case loops its last index over ndims[7]. */
+int zmpo_symAllowed(int rank, int nqnum, int* ndims, int* status, int* allowed, double* a0,
+                    double* a1, double* a2, double* a3, double* a4, double* a5, double* a6,
+                    double* a7);
+
+/* Same rule on integer QNs (exact equality), rank in [1, 16]. */
+int zmpo_symAllowedInt(int rank, int nqnum, const int* ndims, const int* status, int* allowed,
+                       const int32_t* const* qsyms);
+
+/* Grouped block GEMM (device arrays tasks[ntasks], pairs[...]; ntiles = total
+ * tiles).  beta = 0 overwrites each output block, 1 accumulates into it.
+ * dtype ZMPO_F64 (FP64 MFMA) or ZMPO_F32 (FP32 MFMA); A, B, C share it. */
+int zmpo_blockgemm(int dtype, const zmpo_task* tasks, int64_t ntasks, const zmpo_pair* pairs,
+                   int64_t ntiles, const void* A, const void* B, void* C, int beta, void* stream);
+
+/* out[j] = sum_i X[j*ldx + i] * y[i] for j < nx (device out, float64;
+ * work holds zmpo_dots_workspace(nx) doubles).  Fixed reduction order. */
+int zmpo_dots(int dtype, const void* X, int64_t ldx, int nx, const void* y, int64_t n,
+              double* out, double* work, void* stream);
+int zmpo_dots_workspace(int nx);
+
+/* out = beta * out + alpha * sum_j coef[j] * X[j*ldx:...]  (coef: device
+ * float64; beta = 0 ignores the previous contents of out). */
+int zmpo_lincomb(int dtype, void* out, const void* X, int64_t ldx, int nx, const double* coef,
+                 double alpha, double beta, int64_t n, void* stream);
+
+/* r[i] /= |diag[i] - e| + shift  (pdvdson.py:275-278, iop=4). */
+int zmpo_precond(int dtype, void* r, const double* diag, double e, double shift, int64_t n,
+                 void* stream);
+
+/* dst[i] = src[idx[i]] (idx < 0 gives 0) / dst[idx[i]] = src[i]. */
+int zmpo_gather(int dtype_dst, void* dst, int dtype_src, const void* src, const int64_t* idx,
+                int64_t n, void* stream);
+int zmpo_scatter(int dtype_dst, void* dst, int dtype_src, const void* src, const int64_t* idx,
+                 int64_t n, void* stream);
+int zmpo_convert(int dtype_dst, void* dst, int dtype_src, const void* src, int64_t n,
+                 void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZMPO_H */

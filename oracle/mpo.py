@@ -143,7 +143,14 @@ def sort_qnums(qnums):
     The key is numpy's str() of the QN row under this numpy version; the
     reference used its own numpy's format (a block-layout-only hazard).
     """
-    return np.array(sorted(range(len(qnums)), key=lambda k: str(np.asarray(qnums[k]))))
+    memo = {}
+    keys = []
+    for row in qnums:
+        t = np.asarray(row, dtype=np.float64).tobytes()
+        if t not in memo:
+            memo[t] = str(np.asarray(row))
+        keys.append(memo[t])
+    return np.array(sorted(range(len(qnums)), key=lambda k: keys[k]))
 
 
 def mpo_site(p, nsite, isite, int1e, int2e):

@@ -1,0 +1,133 @@
+"""libzmpo kernels on the device vs numpy / the CPU kernel model."""
+import numpy as np
+import pytest
+
+from kmodel import emulate
+from parity_util import Case
+import zmpo_dmrg_amd._lib as L
+from zmpo_dmrg_amd import device as dv
+from zmpo_dmrg_amd import qtensor as pq
+
+pytestmark = pytest.mark.gpu
+
+
+def _dense_program(M, N, K, trans_a=False, trans_b=False):
+    t = np.zeros(1, L.TASK_DTYPE)
+    t[0] = (0, 0, M, N, M, 0, N, 1, 0, 1, 0, 0)
+    p = np.zeros(1, L.PAIR_DTYPE)
+    a_sm, a_sk = (1, M) if trans_a else (K, 1)
+    b_sk, b_sn = (1, K) if trans_b else (N, 1)
+    p[0] = (0, 0, 1.0, K, M, 0, a_sm, a_sk, N, 0, b_sn, b_sk, 0)
+    return pq.Program(t, p)
+
+
+@pytest.mark.parametrize('shape', [(64, 64, 16), (37, 90, 33), (130, 7, 200), (1, 1, 1)])
+@pytest.mark.parametrize('ta,tb', [(False, False), (True, False), (False, True)])
+def test_blockgemm_dense_f64(gpu, shape, ta, tb):
+    M, N, K = shape
+    rng = np.random.default_rng(M * N + K)
+    A = rng.standard_normal(M * K)
+    B = rng.standard_normal(K * N)
+    prog = _dense_program(M, N, K, ta, tb)
+    C = dv.zeros(M * N, 0)
+    pq.run_program(prog, dv.upload(A, 0), dv.upload(B, 0), C, 0)
+    Am = A.reshape(K, M).T if ta else A.reshape(M, K)
+    Bm = B.reshape(N, K).T if tb else B.reshape(K, N)
+    ref = Am @ Bm
+    got = dv.download(C).reshape(M, N)
+    assert np.abs(got - ref).max() < 1e-13 * max(1.0, np.abs(ref).max()) * K
+
+
+@pytest.mark.parametrize('shape', [(64, 64, 16), (37, 90, 33)])
+def test_blockgemm_dense_f32(gpu, shape):
+    M, N, K = shape
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal(M * K).astype(np.float32)
+    B = rng.standard_normal(K * N).astype(np.float32)
+    C = dv.zeros(M * N, 1)
+    pq.run_program(_dense_program(M, N, K), dv.upload(A, 1), dv.upload(B, 1), C, 1)
+    ref = A.reshape(M, K).astype(np.float64) @ B.reshape(K, N).astype(np.float64)
+    assert np.abs(dv.download(C).reshape(M, N) - ref).max() < 1e-5 * K
+
+
+def test_blockgemm_beta_accumulates(gpu):
+    M = N = K = 20
+    rng = np.random.default_rng(2)
+    A, B, C0 = (rng.standard_normal(n) for n in (M * K, K * N, M * N))
+    C = dv.upload(C0, 0)
+    pq.run_program(_dense_program(M, N, K), dv.upload(A, 0), dv.upload(B, 0), C, 0, beta=1)
+    ref = C0.reshape(M, N) + A.reshape(M, K) @ B.reshape(K, N)
+    assert np.abs(dv.download(C).reshape(M, N) - ref).max() < 1e-12
+
+
+@pytest.fixture(scope='module')
+def case4():
+    return Case(K=4, D=16, seed=3, sweeps=1)
+
+
+def test_blockgemm_real_plans_match_model(gpu, case4):
+    """Every stage program of one sigma: device == CPU model (FP64 rounding)."""
+    from kmodel import host_model
+    from zmpo_dmrg_amd import kernelQt as kq
+    with host_model():
+        Lh, Rh = case4.product_envs()
+    isite = 1
+    qkey, _, pt, _ = case4.templates(isite)
+    prog = kq.SigmaProgram(case4.mpo, isite, pt, qkey, Lh[isite - 1], Rh[isite + 2],
+                           case4.mpo.hpwts)
+    rng = np.random.default_rng(0)
+    for st in prog.stages:
+        for k, pg in enumerate(st):
+            if pg.ntasks == 0:
+                continue
+            na = int(max(pg.pairs_host['a_off'].max() + 10000, 1))
+            nb = int(max(pg.pairs_host['b_off'].max() + 10000, 1))
+            nc = int(pg.tasks_host['c_off'].max() + 10000)
+            A, B = rng.standard_normal(na), rng.standard_normal(nb)
+            want = np.zeros(nc)
+            emulate(pg, A, B, want)
+            C = dv.zeros(nc, 0)
+            pq.run_program(pg, dv.upload(A, 0), dv.upload(B, 0), C, 0)
+            assert np.abs(dv.download(C) - want).max() < 1e-12 * max(1.0, np.abs(want).max())
+
+
+def test_vector_kernels(gpu):
+    lib = L.lib()
+    n = 100003
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((3, n))
+    y = rng.standard_normal(n)
+    Xd, yd = dv.upload(X, 0).view(3, n), dv.upload(y, 0)
+    out = dv.zeros(3, 0)
+    work = dv.empty(lib.zmpo_dots_workspace(3), 0)
+    L.check(lib.zmpo_dots(0, dv.ptr(Xd), n, 3, dv.ptr(yd), n, dv.ptr(out), dv.ptr(work), dv.stream()))
+    assert np.abs(dv.download(out) - X @ y).max() < 1e-10
+    c = np.array([0.5, -2.0, 3.0])
+    z = rng.standard_normal(n)
+    zd = dv.upload(z, 0)
+    L.check(lib.zmpo_lincomb(0, dv.ptr(zd), dv.ptr(Xd), n, 3, dv.ptr(dv.upload(c, 0)), -1.5, 2.0,
+                             n, dv.stream()))
+    assert np.abs(dv.download(zd) - (2.0 * z - 1.5 * (c @ X))).max() < 1e-12
+    diag = rng.standard_normal(n)
+    r = rng.standard_normal(n)
+    rd = dv.upload(r, 0)
+    L.check(lib.zmpo_precond(0, dv.ptr(rd), dv.ptr(dv.upload(diag, 0)), 0.3, 1e-4, n, dv.stream()))
+    assert np.abs(dv.download(rd) - r / (np.abs(diag - 0.3) + 1e-4)).max() < 1e-9
+    idx = rng.permutation(n).astype(np.int64)
+    g = dv.zeros(n, 0)
+    pq.gather(g, dv.upload(y, 0), dv.upload(idx))
+    assert np.array_equal(dv.download(g), y[idx])
+    # deterministic reduction: repeated dots are bitwise identical
+    vals = []
+    for _ in range(3):
+        L.check(lib.zmpo_dots(0, dv.ptr(Xd), n, 3, dv.ptr(yd), n, dv.ptr(out), dv.ptr(work),
+                              dv.stream()))
+        vals.append(dv.download(out).copy())
+    assert all(np.array_equal(vals[0], v) for v in vals)
+
+
+def test_errors_surface(gpu):
+    lib = L.lib()
+    rc = lib.zmpo_blockgemm(7, None, 1, None, 1, None, None, None, 0, None)
+    assert rc != 0
+    assert lib.zmpo_last_error()

@@ -1,0 +1,103 @@
+"""Device sigma / diagonal / renormalisation / sweeps vs the FP64 oracle."""
+import os
+
+import numpy as np
+import pytest
+
+from parity_util import Case
+from zmpo_dmrg_amd import device as dv
+from zmpo_dmrg_amd import dmrg as pdmrg
+from zmpo_dmrg_amd import kernelQt as kq
+from zmpo_dmrg_amd.integrals import gen_integrals, gen_spatial, spin_orbital
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
+
+
+@pytest.fixture(scope='module', params=[(4, 16, 3), (5, 24, 4)])
+def case(request):
+    K, D, seed = request.param
+    c = Case(K=K, D=D, seed=seed, sweeps=1, upload=True)
+    c.L, c.R = c.product_envs()
+    c.L[-1] = kq.boundary_store(2 * K)
+    c.R[K] = kq.boundary_store(2 * K)
+    return c
+
+
+def test_envs_on_device_match_oracle(gpu, case):
+    for i in range(case.K):
+        for p in range(2 * case.K):
+            a = case.L[i].op_tensor(p).toDenseTensor()
+            assert np.abs(a - case.oL[i][p].toDenseTensor()).max() < 1e-12
+            b = case.R[i].op_tensor(p).toDenseTensor()
+            assert np.abs(b - case.oR[i][p].toDenseTensor()).max() < 1e-12
+
+
+def test_sigma_and_diag_on_device_match_oracle(gpu, case):
+    for isite in range(case.K - 1):
+        qkey, oq, pt, _ = case.templates(isite)
+        v = np.random.default_rng(isite).standard_normal(oq.size_allowed)
+        want = case.oracle_sigma(isite, v)
+        prog = kq.SigmaProgram(case.mpo, isite, pt, qkey, case.L[isite - 1], case.R[isite + 2],
+                               case.mpo.hpwts)
+        sig = dv.empty(pt.size_allowed, 0)
+        prog.apply(case.mpo, case.L[isite - 1], case.R[isite + 2], dv.upload(v, 0), sig)
+        got = dv.download(sig)
+        assert np.abs(got - want).max() < 1e-12 * max(1.0, np.abs(want).max()), isite
+        d = kq.DiagProgram(case.mpo, isite, pt, case.L[isite - 1], case.R[isite + 2],
+                           case.mpo.hpwts)
+        dg = dv.zeros(pt.size_allowed, 0)
+        d.apply(case.L[isite - 1], case.R[isite + 2], dg)
+        assert np.abs(dv.download(dg) - case.oracle_diag(isite)).max() < 1e-12
+
+
+def test_sigma_fp32_within_tolerance(gpu, case):
+    isite = 1
+    qkey, oq, pt, _ = case.templates(isite)
+    v = np.random.default_rng(9).standard_normal(oq.size_allowed)
+    want = case.oracle_sigma(isite, v)
+    prog = kq.SigmaProgram(case.mpo, isite, pt, qkey, case.L[isite - 1], case.R[isite + 2],
+                           case.mpo.hpwts)
+    sig = dv.empty(pt.size_allowed, 1)
+    prog.apply(case.mpo, case.L[isite - 1], case.R[isite + 2], dv.upload(v, 1), sig, dtype=1)
+    got = dv.download(sig).astype(np.float64)
+    assert np.abs(got - want).max() < 1e-5 * max(1.0, np.abs(want).max())
+
+
+def test_device_dmrg_equals_fci(gpu):
+    from oracle import fci
+    K = 4
+    i1, i2 = gen_integrals(K, seed=1)
+    h, eri = gen_spatial(K, seed=1)
+    d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=100, tol=1e-10)
+    e = pdmrg.run(d, 4)
+    assert abs(e[-1] - fci.fci_energy(h, eri, K, 0.0)[0]) < 1e-8
+
+
+def test_device_dmrg_h6_reference_energy(gpu):
+    """H6/STO-3G from the reference's example: FCI -5.1491130015 (README:2)."""
+    z = np.load(os.path.join(GOLDEN, 'h6_sto3g.npz'))
+    i1, i2 = spin_orbital(z['h'], z['eri'])
+    d = pdmrg.MPODMRG(i1, i2, 6, 0.0, Dmax=64, tol=1e-10, occun=np.array([1., 0., 0., 1.] * 3))
+    e = pdmrg.run(d, 6)
+    assert abs(e[-1] - (-5.1491130015)) < 2e-10
+
+
+def test_device_dmrg_matches_oracle_sweep_by_sweep(gpu):
+    """Truncated run (D below the full space): energies per half sweep and the
+    bond QN structure after truncation identical to the oracle."""
+    from oracle import sweep as osw
+    K, D = 5, 12
+    i1, i2 = gen_integrals(K, seed=5)
+    st = osw.State(i1, i2, K, 0.0, Dmax=D, tol=1e-10)
+    d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=D, tol=1e-10)
+    for _ in range(2):
+        for status in ('R', 'L'):
+            ro = osw.sweep(st, status)
+            rp = pdmrg.sweep(d, status)
+            eo = np.array([x[0] for x in ro[3]])
+            ep = np.array([x[0] for x in rp[3]])
+            assert np.abs(eo - ep).max() < 1e-9
+            for i in range(K + 1):
+                assert np.array_equal(np.asarray(st.qnuml[i]), np.asarray(d.qnuml[i]))
+                assert np.array_equal(np.asarray(st.qnumr[i]), np.asarray(d.qnumr[i]))

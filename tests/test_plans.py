@@ -1,0 +1,101 @@
+"""Plan lowering and engine wiring, executed by the CPU model of the block
+GEMM (tests/kmodel.py): zmpo_dmrg_amd programs vs the oracle, no device."""
+import numpy as np
+import pytest
+
+from kmodel import emulate, host_model
+from oracle import mpo as ompo
+from parity_util import Case, to_product
+from zmpo_dmrg_amd import kernelQt as kq
+from zmpo_dmrg_amd import mpo as pmpo
+from zmpo_dmrg_amd import qtensor as pq
+from zmpo_dmrg_amd.integrals import gen_integrals
+
+
+@pytest.mark.parametrize('K', [3, 5])
+def test_mpo_bitwise_equal_to_oracle(K):
+    i1, i2 = gen_integrals(K, seed=7)
+    ms = pmpo.MPOStore(i1, i2, upload=False)
+    om = ompo.build_mpo(i1, i2)
+    for i in range(K):
+        for p in range(2 * K):
+            a, b = ms.sites[i][p], om[i][p]
+            assert np.array_equal(a.idx_allowed, b.idx_allowed)
+            assert all(np.array_equal(x, y) for x, y in zip(a.qsyms, b.qsyms))
+            assert np.array_equal(a.value, b.value)
+
+
+@pytest.fixture(scope='module')
+def case4():
+    return Case(K=4, D=16, seed=3, sweeps=1)
+
+
+def test_env_build_matches_oracle(case4):
+    with host_model():
+        L, R = case4.product_envs()
+        for i in range(case4.K):
+            for p in range(2 * case4.K):
+                a = L[i].op_tensor(p).toDenseTensor()
+                assert np.abs(a - case4.oL[i][p].toDenseTensor()).max() < 1e-13
+                b = R[i].op_tensor(p).toDenseTensor()
+                assert np.abs(b - case4.oR[i][p].toDenseTensor()).max() < 1e-13
+
+
+@pytest.mark.parametrize('isite', [0, 1, 2])
+def test_sigma_and_diag_match_oracle(case4, isite):
+    with host_model():
+        L, R = case4.product_envs()
+        L[-1] = kq.boundary_store(2 * case4.K)
+        R[case4.K] = kq.boundary_store(2 * case4.K)
+        qkey, oq, pt, _ = case4.templates(isite)
+        v = np.random.default_rng(isite).standard_normal(oq.size_allowed)
+        want = case4.oracle_sigma(isite, v)
+        prog = kq.SigmaProgram(case4.mpo, isite, pt, qkey, L[isite - 1], R[isite + 2],
+                               case4.mpo.hpwts)
+        got = np.zeros(pt.size_allowed)
+        prog.apply(case4.mpo, L[isite - 1], R[isite + 2], v, got)
+        assert np.abs(got - want).max() < 1e-12 * max(1.0, np.abs(want).max())
+        d = kq.DiagProgram(case4.mpo, isite, pt, L[isite - 1], R[isite + 2], case4.mpo.hpwts)
+        dg = np.zeros(pt.size_allowed)
+        d.apply(L[isite - 1], R[isite + 2], dg)
+        assert np.abs(dg - case4.oracle_diag(isite)).max() < 1e-12
+
+
+def test_sigma_chunking_is_order_stable(case4):
+    """Op chunks (bounded intermediates) give the same sigma as one chunk."""
+    with host_model():
+        L, R = case4.product_envs()
+        isite = 1
+        qkey, oq, pt, _ = case4.templates(isite)
+        v = np.random.default_rng(0).standard_normal(oq.size_allowed)
+        out = []
+        for budget in (1 << 40, 4096):
+            prog = kq.SigmaProgram(case4.mpo, isite, pt, qkey, L[isite - 1], R[isite + 2],
+                                   case4.mpo.hpwts, budget=budget)
+            got = np.zeros(pt.size_allowed)
+            prog.apply(case4.mpo, L[isite - 1], R[isite + 2], v, got)
+            out.append((len(prog.chunks), got))
+        assert out[1][0] > 1
+        assert np.abs(out[0][1] - out[1][1]).max() < 1e-12
+
+
+def test_generic_tensordot_lowering():
+    """Reference-layout tensordot for several axes patterns vs numpy on dense."""
+    rng = np.random.default_rng(1)
+    qp = np.array([[0., 0.], [1., -.5], [1., .5], [2., 0.]])
+    qb = qp[rng.integers(0, 4, 7)]
+    a = pq.QTensor([False, False, True])
+    a.fromDenseTensor(rng.standard_normal((7, 4, 7)), [qb, qp, qb + qb[::-1] * 0])
+    b = pq.QTensor([False, False, True])
+    b.fromDenseTensor(rng.standard_normal((7, 4, 7)), [qb, qp, qb])
+    with host_model():
+        for axes in (([0], [0]), ([0, 1], [0, 1]), ([2], [0])):
+            try:
+                out = pq.QTensor()
+                out.tensordotSYM(a, b, axes)
+            except (AssertionError, ValueError):
+                continue
+            out.tensordotCAL(a, b)
+            il = [a.idlst[i] for i in out.plan.e1] + [b.idlst[i] for i in out.plan.e2]
+            ref = np.tensordot(a.toDenseTensor(), b.toDenseTensor(), axes)
+            assert np.abs(out.toDenseTensor(il) - ref).max() < 1e-12

@@ -1,0 +1,93 @@
+"""ctypes binding of libzmpo (include/zmpo.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``python -m zmpo_dmrg_amd.build``).  Nothing here falls back to a CPU
+implementation: a missing library or a failing call raises ZmpoError.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libzmpo.so')
+
+F64 = 0
+F32 = 1
+
+TASK_DTYPE = np.dtype([('c_off', '<i8'), ('tile_base', '<i8'), ('M', '<i4'), ('N', '<i4'),
+                       ('c_mlo', '<i4'), ('c_smhi', '<i4'), ('c_smlo', '<i4'), ('c_sn', '<i4'),
+                       ('p_begin', '<i4'), ('p_end', '<i4'), ('tiles_n', '<i4'), ('pad', '<i4')])
+PAIR_DTYPE = np.dtype([('a_off', '<i8'), ('b_off', '<i8'), ('scale', '<f8'), ('K', '<i4'),
+                       ('a_mlo', '<i4'), ('a_smhi', '<i4'), ('a_smlo', '<i4'), ('a_sk', '<i4'),
+                       ('b_nlo', '<i4'), ('b_snhi', '<i4'), ('b_snlo', '<i4'), ('b_sk', '<i4'),
+                       ('pad', '<i4')])
+TILE = 64
+
+EXPORTS = ['zmpo_last_error', 'zmpo_abi_version', 'zmpo_struct_sizes', 'zmpo_symAllowed',
+           'zmpo_symAllowedInt', 'zmpo_blockgemm', 'zmpo_dots', 'zmpo_dots_workspace',
+           'zmpo_lincomb', 'zmpo_precond', 'zmpo_gather', 'zmpo_scatter', 'zmpo_convert']
+
+
+class ZmpoError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load (once) and return the ctypes handle; raises if the library is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ZmpoError('libzmpo.so not built at %s: run __graft_entry__.build()' % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, i32, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+    L.zmpo_last_error.restype = ctypes.c_char_p
+    L.zmpo_abi_version.restype = i32
+    L.zmpo_struct_sizes.argtypes = [ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.zmpo_symAllowed.argtypes = [i32, i32, vp, vp, vp] + [vp] * 8
+    L.zmpo_symAllowedInt.argtypes = [i32, i32, vp, vp, vp, vp]
+    L.zmpo_blockgemm.argtypes = [i32, vp, i64, vp, i64, vp, vp, vp, i32, vp]
+    L.zmpo_dots.argtypes = [i32, vp, i64, i32, vp, i64, vp, vp, vp]
+    L.zmpo_dots_workspace.argtypes = [i32]
+    L.zmpo_dots_workspace.restype = i32
+    L.zmpo_lincomb.argtypes = [i32, vp, vp, i64, i32, vp, dbl, dbl, i64, vp]
+    L.zmpo_precond.argtypes = [i32, vp, vp, dbl, dbl, i64, vp]
+    L.zmpo_gather.argtypes = [i32, vp, i32, vp, vp, i64, vp]
+    L.zmpo_scatter.argtypes = [i32, vp, i32, vp, vp, i64, vp]
+    L.zmpo_convert.argtypes = [i32, vp, i32, vp, i64, vp]
+    ts, ps = ctypes.c_int(), ctypes.c_int()
+    L.zmpo_struct_sizes(ctypes.byref(ts), ctypes.byref(ps))
+    if ts.value != TASK_DTYPE.itemsize or ps.value != PAIR_DTYPE.itemsize:
+        raise ZmpoError('ABI struct size mismatch: %d/%d vs %d/%d' % (
+            ts.value, ps.value, TASK_DTYPE.itemsize, PAIR_DTYPE.itemsize))
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise ZmpoError(lib().zmpo_last_error().decode())
+    return rc
+
+
+def sym_allowed(nsyms, status, qsyms):
+    """Allowed-block mask (int32, C order) through zmpo_symAllowed (qsym.c:5-22)."""
+    rank = len(nsyms)
+    if rank == 1:
+        q = np.asarray(qsyms[0], dtype=np.float64)
+        sgn = -1.0 if status[0] else 1.0
+        return (np.abs(sgn * q).sum(axis=1) < 1e-10).astype(np.int32)
+    if rank > 8:
+        raise ZmpoError('block rule supports rank <= 8')
+    nq = int(np.asarray(qsyms[0]).shape[1])
+    nd = np.ascontiguousarray(nsyms, dtype=np.int32)
+    st = np.ascontiguousarray([1 if s else 0 for s in status], dtype=np.int32)
+    out = np.zeros(int(np.prod(nd)), dtype=np.int32)
+    qs = [np.ascontiguousarray(q, dtype=np.float64) for q in qsyms]
+    ptrs = [q.ctypes.data for q in qs] + [None] * (8 - rank)
+    check(lib().zmpo_symAllowed(rank, nq, nd.ctypes.data, st.ctypes.data, out.ctypes.data, *ptrs))
+    return out

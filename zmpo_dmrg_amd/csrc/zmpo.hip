@@ -1,0 +1,506 @@
+// libzmpo: MI355X (gfx950) kernels and C-ABI for the two-site sigma-vector,
+// environment renormalisation and Davidson vector work of ZMPO-DMRG.
+//
+// The hot op is a grouped block GEMM: one launch executes every block pair
+// of one or more block-sparse contraction plans.  A "task" is one output
+// block; a workgroup owns one 64x64 tile of one task and loops over all of
+// that task's pairs, so each output element is written exactly once, in a
+// fixed order (deterministic, no atomics).  Block-internal index maps are
+// 2-level (m = mhi*mlo_dim + mlo) so permuted legs are folded into the
+// LDS-staged loads instead of being materialised by transpose copies.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/zmpo.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char* what) {
+  g_last_error = what;
+  return code;
+}
+
+int check_hip(hipError_t e, const char* where) {
+  if (e == hipSuccess) return ZMPO_OK;
+  g_last_error = std::string(where) + ": " + hipGetErrorString(e);
+  return ZMPO_ERR_HIP;
+}
+
+static_assert(sizeof(zmpo_task) == 56, "zmpo_task layout");
+static_assert(sizeof(zmpo_pair) == 64, "zmpo_pair layout");
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Acc;
+template <>
+struct Acc<double> {
+  typedef dbl4 type;
+  __device__ static inline dbl4 mfma(double a, double b, dbl4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Acc<float> {
+  typedef flt4 type;
+  __device__ static inline flt4 mfma(float a, float b, flt4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int TM = 64;
+constexpr int TN = 64;
+constexpr int TK = 16;
+constexpr int NTHREADS = 256;
+
+// Locate the task owning global tile t (tasks sorted by tile_base).
+__device__ inline int find_task(const zmpo_task* __restrict__ tasks, int ntasks, int64_t t) {
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].tile_base <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// C[task] (=|+=) sum_pairs scale * A_pair(M x K) * B_pair(K x N).
+// 4 waves in a 2x2 grid, each wave owns a 32x32 quadrant = 2x2 MFMA 16x16
+// accumulators.  MFMA 16x16x4 operand layout: lane l holds A[l%16][l/16] and
+// B[l/16][l%16]; result element r of lane l is D[4*(l/16)+r][l%16].
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void blockgemm_kernel(
+    const zmpo_task* __restrict__ tasks, int ntasks, const zmpo_pair* __restrict__ pairs,
+    const T* __restrict__ A, const T* __restrict__ B, T* __restrict__ C, int beta,
+    int64_t ntiles) {
+  typedef typename Acc<T>::type accv;
+  __shared__ T As[TK][TM + 1];
+  __shared__ T Bs[TK][TN + 1];
+  __shared__ int64_t a_moff[TM];
+  __shared__ int64_t b_noff[TN];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1;
+  const int wn = wave & 1;
+  const int lrow = lane & 15;
+  const int lk = lane >> 4;
+
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const zmpo_task tk = tasks[find_task(tasks, ntasks, t)];
+    const int64_t local = t - tk.tile_base;
+    const int m0 = (int)(local / tk.tiles_n) * TM;
+    const int n0 = (int)(local % tk.tiles_n) * TN;
+    accv acc[2][2];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) acc[i][j] = accv{0, 0, 0, 0};
+
+    for (int p = tk.p_begin; p < tk.p_end; ++p) {
+      const zmpo_pair pr = pairs[p];
+      const T scale = (T)pr.scale;
+      __syncthreads();
+      if (tid < TM) {
+        const int m = m0 + tid;
+        a_moff[tid] = (m < tk.M)
+                          ? (int64_t)(m / pr.a_mlo) * pr.a_smhi + (int64_t)(m % pr.a_mlo) * pr.a_smlo
+                          : (int64_t)-1;
+      } else if (tid < TM + TN) {
+        const int i = tid - TM;
+        const int n = n0 + i;
+        b_noff[i] = (n < tk.N)
+                        ? (int64_t)(n / pr.b_nlo) * pr.b_snhi + (int64_t)(n % pr.b_nlo) * pr.b_snlo
+                        : (int64_t)-1;
+      }
+      __syncthreads();
+      const T* __restrict__ Ap = A + pr.a_off;
+      const T* __restrict__ Bp = B + pr.b_off;
+      for (int k0 = 0; k0 < pr.K; k0 += TK) {
+        if (pr.a_sk == 1) {
+          // K-contiguous rows: consecutive threads walk k.
+          for (int e = tid; e < TM * TK; e += NTHREADS) {
+            const int kk = e % TK, mm = e / TK;
+            const int k = k0 + kk;
+            const int64_t mo = a_moff[mm];
+            T v = 0;
+            if (mo >= 0 && k < pr.K) v = Ap[mo + k];
+            As[kk][mm] = v * scale;
+          }
+        } else {
+          for (int e = tid; e < TM * TK; e += NTHREADS) {
+            const int mm = e % TM, kk = e / TM;
+            const int k = k0 + kk;
+            const int64_t mo = a_moff[mm];
+            T v = 0;
+            if (mo >= 0 && k < pr.K) v = Ap[mo + (int64_t)k * pr.a_sk];
+            As[kk][mm] = v * scale;
+          }
+        }
+        if (pr.b_sk == 1) {
+          for (int e = tid; e < TN * TK; e += NTHREADS) {
+            const int kk = e % TK, nn = e / TK;
+            const int k = k0 + kk;
+            const int64_t no = b_noff[nn];
+            T v = 0;
+            if (no >= 0 && k < pr.K) v = Bp[no + k];
+            Bs[kk][nn] = v;
+          }
+        } else {
+          for (int e = tid; e < TN * TK; e += NTHREADS) {
+            const int nn = e % TN, kk = e / TN;
+            const int k = k0 + kk;
+            const int64_t no = b_noff[nn];
+            T v = 0;
+            if (no >= 0 && k < pr.K) v = Bp[no + (int64_t)k * pr.b_sk];
+            Bs[kk][nn] = v;
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < TK; kk += 4) {
+          const T a0 = As[kk + lk][wm * 32 + lrow];
+          const T a1 = As[kk + lk][wm * 32 + 16 + lrow];
+          const T b0 = Bs[kk + lk][wn * 32 + lrow];
+          const T b1 = Bs[kk + lk][wn * 32 + 16 + lrow];
+          acc[0][0] = Acc<T>::mfma(a0, b0, acc[0][0]);
+          acc[0][1] = Acc<T>::mfma(a0, b1, acc[0][1]);
+          acc[1][0] = Acc<T>::mfma(a1, b0, acc[1][0]);
+          acc[1][1] = Acc<T>::mfma(a1, b1, acc[1][1]);
+        }
+        __syncthreads();
+      }
+    }
+    // Epilogue: one owner per output element.
+    for (int rb = 0; rb < 2; ++rb) {
+      for (int r = 0; r < 4; ++r) {
+        const int mi = m0 + wm * 32 + rb * 16 + 4 * lk + r;
+        if (mi >= tk.M) continue;
+        const int64_t moff = tk.c_off + (int64_t)(mi / tk.c_mlo) * tk.c_smhi +
+                             (int64_t)(mi % tk.c_mlo) * tk.c_smlo;
+        for (int cb = 0; cb < 2; ++cb) {
+          const int ni = n0 + wn * 32 + cb * 16 + lrow;
+          if (ni >= tk.N) continue;
+          const int64_t off = moff + (int64_t)ni * tk.c_sn;
+          const T v = acc[rb][cb][r];
+          C[off] = beta ? C[off] + v : v;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- vectors
+// out[j] = sum_i X[j*ldx + i] * y[i], j < nx.  Deterministic two-level
+// reduction: block b sums a fixed contiguous chunk, the second pass adds the
+// per-block partials in block order.
+constexpr int RED_THREADS = 256;
+constexpr int RED_BLOCKS = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void dots_partial_kernel(const T* __restrict__ X,
+                                                                   int64_t ldx, int nx,
+                                                                   const T* __restrict__ y,
+                                                                   int64_t n, double* __restrict__ part) {
+  __shared__ double red[RED_THREADS];
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t beg = (int64_t)blockIdx.x * chunk;
+  const int64_t end = beg + chunk < n ? beg + chunk : n;
+  for (int j = 0; j < nx; ++j) {
+    double s = 0.0;
+    const T* __restrict__ xj = X + (int64_t)j * ldx;
+    for (int64_t i = beg + threadIdx.x; i < end; i += RED_THREADS) s += (double)xj[i] * (double)y[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)j * gridDim.x + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void dots_final_kernel(const double* __restrict__ part,
+                                                                int nblk, int nx,
+                                                                double* __restrict__ out) {
+  __shared__ double red[RED_THREADS];
+  const int j = blockIdx.x;
+  if (j >= nx) return;
+  double s = 0.0;
+  // fixed order: thread t sums blocks t, t+256, ... then tree-reduce
+  for (int b = threadIdx.x; b < nblk; b += RED_THREADS) s += part[(int64_t)j * nblk + b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = red[0];
+}
+
+// out[i] = beta*out[i] + alpha * sum_j c[j] * X[j*ldx + i]   (c on the device)
+template <typename T>
+__global__ void lincomb_kernel(T* __restrict__ out, const T* __restrict__ X, int64_t ldx, int nx,
+                               const double* __restrict__ c, double alpha, double beta,
+                               int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int j = 0; j < nx; ++j) s += c[j] * (double)X[(int64_t)j * ldx + i];
+    out[i] = (T)(beta == 0.0 ? alpha * s : beta * (double)out[i] + alpha * s);
+  }
+}
+
+// r[i] /= |diag[i] - e| + shift   (Davidson iop=4 preconditioner)
+template <typename T>
+__global__ void precond_kernel(T* __restrict__ r, const double* __restrict__ diag, double e,
+                               double shift, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    r[i] = (T)((double)r[i] / (fabs(diag[i] - e) + shift));
+}
+
+// dst[i] = src[idx[i]] (idx < 0 -> 0)
+template <typename TD, typename TS>
+__global__ void gather_kernel(TD* __restrict__ dst, const TS* __restrict__ src,
+                              const int64_t* __restrict__ idx, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = idx[i];
+    dst[i] = j >= 0 ? (TD)src[j] : (TD)0;
+  }
+}
+
+// dst[idx[i]] = src[i]  (idx must be injective)
+template <typename TD, typename TS>
+__global__ void scatter_kernel(TD* __restrict__ dst, const TS* __restrict__ src,
+                               const int64_t* __restrict__ idx, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[idx[i]] = (TD)src[i];
+}
+
+template <typename TD, typename TS>
+__global__ void convert_kernel(TD* __restrict__ dst, const TS* __restrict__ src, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (TD)src[i];
+}
+
+inline unsigned grid_for(int64_t n, int threads) {
+  int64_t g = (n + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (unsigned)g;
+}
+
+template <typename T>
+int launch_blockgemm(const zmpo_task* tasks, int64_t ntasks, const zmpo_pair* pairs,
+                     int64_t ntiles, const void* A, const void* B, void* C, int beta,
+                     hipStream_t s) {
+  if (ntiles <= 0 || ntasks <= 0) return ZMPO_OK;
+  const int64_t cap = (int64_t)1 << 30;
+  const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
+  hipLaunchKernelGGL(blockgemm_kernel<T>, dim3(grid), dim3(NTHREADS), 0, s, tasks, (int)ntasks,
+                     pairs, (const T*)A, (const T*)B, (T*)C, beta, ntiles);
+  return check_hip(hipGetLastError(), "blockgemm launch");
+}
+
+// Reference-compatible block rule for one rank (qsym.c:24-75 semantics).
+template <typename Q>
+void sym_allowed_impl(int rank, int nqnum, const int* ndims, const int* status, int* allowed,
+                      const Q* const* q, double tol) {
+  int64_t nblk = 1;
+  for (int i = 0; i < rank; ++i) nblk *= ndims[i];
+  int idx[16] = {0};
+  double diff[16];
+  for (int64_t b = 0; b < nblk; ++b) {
+    for (int iq = 0; iq < nqnum; ++iq) diff[iq] = 0.0;
+    for (int r = 0; r < rank; ++r) {
+      const Q* row = q[r] + (int64_t)idx[r] * nqnum;
+      for (int iq = 0; iq < nqnum; ++iq)
+        diff[iq] += status[r] == 0 ? (double)row[iq] : -(double)row[iq];
+    }
+    double tot = 0.0;
+    for (int iq = 0; iq < nqnum; ++iq) tot += fabs(diff[iq]);
+    allowed[b] = tot < tol;
+    for (int r = rank - 1; r >= 0; --r) {  // C-order odometer
+      if (++idx[r] < ndims[r]) break;
+      idx[r] = 0;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* zmpo_last_error(void) { return g_last_error.c_str(); }
+
+int zmpo_abi_version(void) { return ZMPO_ABI_VERSION; }
+
+int zmpo_struct_sizes(int* task_size, int* pair_size) {
+  if (!task_size || !pair_size) return set_error(ZMPO_ERR_ARG, "null pointer");
+  *task_size = (int)sizeof(zmpo_task);
+  *pair_size = (int)sizeof(zmpo_pair);
+  return ZMPO_OK;
+}
+
+int zmpo_symAllowed(int rank, int nqnum, int* ndims, int* status, int* allowed, double* a0,
+                    double* a1, double* a2, double* a3, double* a4, double* a5, double* a6,
+                    double* a7) {
+  if (rank < 2 || rank > 8) return set_error(ZMPO_ERR_ARG, "symAllowed: rank must be in [2,8]");
+  if (nqnum < 1 || nqnum > 16) return set_error(ZMPO_ERR_ARG, "symAllowed: nqnum out of range");
+  if (!ndims || !status || !allowed) return set_error(ZMPO_ERR_ARG, "symAllowed: null pointer");
+  const double* q[8] = {a0, a1, a2, a3, a4, a5, a6, a7};
+  for (int i = 0; i < rank; ++i)
+    if (!q[i]) return set_error(ZMPO_ERR_ARG, "symAllowed: null QN array");
+  sym_allowed_impl<double>(rank, nqnum, ndims, status, allowed, q, 1e-10);
+  return ZMPO_OK;
+}
+
+int zmpo_symAllowedInt(int rank, int nqnum, const int* ndims, const int* status, int* allowed,
+                       const int32_t* const* qsyms) {
+  if (rank < 1 || rank > 16) return set_error(ZMPO_ERR_ARG, "symAllowedInt: rank must be in [1,16]");
+  if (nqnum < 1 || nqnum > 16) return set_error(ZMPO_ERR_ARG, "symAllowedInt: nqnum out of range");
+  if (!ndims || !status || !allowed || !qsyms) return set_error(ZMPO_ERR_ARG, "symAllowedInt: null pointer");
+  for (int i = 0; i < rank; ++i)
+    if (!qsyms[i]) return set_error(ZMPO_ERR_ARG, "symAllowedInt: null QN array");
+  sym_allowed_impl<int32_t>(rank, nqnum, ndims, status, allowed, qsyms, 0.5);
+  return ZMPO_OK;
+}
+
+int zmpo_blockgemm(int dtype, const zmpo_task* tasks, int64_t ntasks, const zmpo_pair* pairs,
+                   int64_t ntiles, const void* A, const void* B, void* C, int beta,
+                   void* stream) {
+  if (ntasks > 0 && (!tasks || !pairs || !A || !B || !C))
+    return set_error(ZMPO_ERR_ARG, "blockgemm: null pointer");
+  if (ntasks > 0x7fffffff) return set_error(ZMPO_ERR_ARG, "blockgemm: too many tasks");
+  hipStream_t s = (hipStream_t)stream;
+  switch (dtype) {
+    case ZMPO_F64: return launch_blockgemm<double>(tasks, ntasks, pairs, ntiles, A, B, C, beta, s);
+    case ZMPO_F32: return launch_blockgemm<float>(tasks, ntasks, pairs, ntiles, A, B, C, beta, s);
+    default: return set_error(ZMPO_ERR_ARG, "blockgemm: unsupported dtype");
+  }
+}
+
+int zmpo_dots(int dtype, const void* X, int64_t ldx, int nx, const void* y, int64_t n,
+              double* out, double* work, void* stream) {
+  if (nx <= 0) return ZMPO_OK;
+  if (!X || !y || !out || !work) return set_error(ZMPO_ERR_ARG, "dots: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const int nblk = RED_BLOCKS;
+  if (dtype == ZMPO_F64)
+    hipLaunchKernelGGL(dots_partial_kernel<double>, dim3(nblk), dim3(RED_THREADS), 0, s,
+                       (const double*)X, ldx, nx, (const double*)y, n, work);
+  else if (dtype == ZMPO_F32)
+    hipLaunchKernelGGL(dots_partial_kernel<float>, dim3(nblk), dim3(RED_THREADS), 0, s,
+                       (const float*)X, ldx, nx, (const float*)y, n, work);
+  else
+    return set_error(ZMPO_ERR_ARG, "dots: unsupported dtype");
+  int rc = check_hip(hipGetLastError(), "dots partial");
+  if (rc) return rc;
+  hipLaunchKernelGGL(dots_final_kernel, dim3(nx), dim3(RED_THREADS), 0, s, work, nblk, nx, out);
+  return check_hip(hipGetLastError(), "dots final");
+}
+
+int zmpo_dots_workspace(int nx) { return nx * RED_BLOCKS; }
+
+int zmpo_lincomb(int dtype, void* out, const void* X, int64_t ldx, int nx, const double* coef,
+                 double alpha, double beta, int64_t n, void* stream) {
+  if (!out || (nx > 0 && (!X || !coef))) return set_error(ZMPO_ERR_ARG, "lincomb: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == ZMPO_F64)
+    hipLaunchKernelGGL(lincomb_kernel<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (double*)out, (const double*)X, ldx, nx, coef, alpha, beta, n);
+  else if (dtype == ZMPO_F32)
+    hipLaunchKernelGGL(lincomb_kernel<float>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (float*)out, (const float*)X, ldx, nx, coef, alpha, beta, n);
+  else
+    return set_error(ZMPO_ERR_ARG, "lincomb: unsupported dtype");
+  return check_hip(hipGetLastError(), "lincomb");
+}
+
+int zmpo_precond(int dtype, void* r, const double* diag, double e, double shift, int64_t n,
+                 void* stream) {
+  if (!r || !diag) return set_error(ZMPO_ERR_ARG, "precond: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == ZMPO_F64)
+    hipLaunchKernelGGL(precond_kernel<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (double*)r, diag, e, shift, n);
+  else if (dtype == ZMPO_F32)
+    hipLaunchKernelGGL(precond_kernel<float>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (float*)r, diag, e, shift, n);
+  else
+    return set_error(ZMPO_ERR_ARG, "precond: unsupported dtype");
+  return check_hip(hipGetLastError(), "precond");
+}
+
+int zmpo_gather(int dtype_dst, void* dst, int dtype_src, const void* src, const int64_t* idx,
+                int64_t n, void* stream) {
+  if (n <= 0) return ZMPO_OK;
+  if (!dst || !src || !idx) return set_error(ZMPO_ERR_ARG, "gather: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype_dst == ZMPO_F64 && dtype_src == ZMPO_F64)
+    hipLaunchKernelGGL((gather_kernel<double, double>), dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (double*)dst, (const double*)src, idx, n);
+  else if (dtype_dst == ZMPO_F32 && dtype_src == ZMPO_F64)
+    hipLaunchKernelGGL((gather_kernel<float, double>), dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (float*)dst, (const double*)src, idx, n);
+  else if (dtype_dst == ZMPO_F64 && dtype_src == ZMPO_F32)
+    hipLaunchKernelGGL((gather_kernel<double, float>), dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (double*)dst, (const float*)src, idx, n);
+  else if (dtype_dst == ZMPO_F32 && dtype_src == ZMPO_F32)
+    hipLaunchKernelGGL((gather_kernel<float, float>), dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (float*)dst, (const float*)src, idx, n);
+  else
+    return set_error(ZMPO_ERR_ARG, "gather: unsupported dtype");
+  return check_hip(hipGetLastError(), "gather");
+}
+
+int zmpo_scatter(int dtype_dst, void* dst, int dtype_src, const void* src, const int64_t* idx,
+                 int64_t n, void* stream) {
+  if (n <= 0) return ZMPO_OK;
+  if (!dst || !src || !idx) return set_error(ZMPO_ERR_ARG, "scatter: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype_dst == ZMPO_F64 && dtype_src == ZMPO_F64)
+    hipLaunchKernelGGL((scatter_kernel<double, double>), dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (double*)dst, (const double*)src, idx, n);
+  else if (dtype_dst == ZMPO_F32 && dtype_src == ZMPO_F64)
+    hipLaunchKernelGGL((scatter_kernel<float, double>), dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (float*)dst, (const double*)src, idx, n);
+  else if (dtype_dst == ZMPO_F64 && dtype_src == ZMPO_F32)
+    hipLaunchKernelGGL((scatter_kernel<double, float>), dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (double*)dst, (const float*)src, idx, n);
+  else if (dtype_dst == ZMPO_F32 && dtype_src == ZMPO_F32)
+    hipLaunchKernelGGL((scatter_kernel<float, float>), dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (float*)dst, (const float*)src, idx, n);
+  else
+    return set_error(ZMPO_ERR_ARG, "scatter: unsupported dtype");
+  return check_hip(hipGetLastError(), "scatter");
+}
+
+int zmpo_convert(int dtype_dst, void* dst, int dtype_src, const void* src, int64_t n,
+                 void* stream) {
+  if (n <= 0) return ZMPO_OK;
+  if (!dst || !src) return set_error(ZMPO_ERR_ARG, "convert: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype_dst == ZMPO_F32 && dtype_src == ZMPO_F64)
+    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (float*)dst, (const double*)src, n);
+  else if (dtype_dst == ZMPO_F64 && dtype_src == ZMPO_F32)
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (double*)dst, (const float*)src, n);
+  else
+    return set_error(ZMPO_ERR_ARG, "convert: unsupported dtype pair");
+  return check_hip(hipGetLastError(), "convert");
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+"""Two-site sweep driver over the device kernels.
+
+Counterpart of source/mpo_dmrg_class.py (mpo_dmrg fields :47-192, build
+:279-297, default :329-434) and source/mpo_dmrg_opt.py (sweep :31-95,
+checkOpers :153-193, sweep_updateSite :313-344, sweep_updateBoundary
+:347-377), with the 'Fixed2' schedule (mpo_dmrg_schedule.py:100-122), noise=0
+and one process.  Environments and the MPO live in device memory (EnvStore /
+MPOStore) instead of HDF5 files; the Qt MPS starts from an occupation
+product state (mpo_dmrg_util.py:118-155) followed by initOps (:193-214).
+"""
+import time
+
+import numpy as np
+
+from . import kernelQt
+from .dot import ci_solver
+from .mpo import MPOStore, qphys
+from .qtensor import QTensor
+
+
+def default_occupation(K, nelec, sz=0.0):
+    """Aufbau occupation, alpha on even / beta on odd spin orbitals."""
+    na = int(round(nelec / 2 + sz))
+    nb = int(round(nelec / 2 - sz))
+    occ = np.zeros(2 * K)
+    occ[0:2 * na:2] = 1.0
+    occ[1:2 * nb:2] = 1.0
+    return occ
+
+
+def mps0occun(K, occun):
+    """Product-state MPS from an occupation (mpo_dmrg_util.py:118-155)."""
+    qnuml = [np.array([[0., 0.]])]
+    sites = []
+    ne, ms = 0.0, 0.0
+    for i in range(K):
+        na, nb = occun[2 * i], occun[2 * i + 1]
+        t = np.zeros(4)
+        if na < 0.5 and nb < 0.5:
+            t[0] = 1.0
+        elif na < 0.5:
+            ne, ms = ne + 1.0, ms - 0.5
+            t[1] = 1.0
+        elif nb < 0.5:
+            ne, ms = ne + 1.0, ms + 0.5
+            t[2] = 1.0
+        else:
+            ne += 2.0
+            t[3] = 1.0
+        sites.append(t.reshape(1, 4, 1))
+        qnuml.append(np.array([[ne, ms]]))
+    qnumr = [np.array([[ne - q[0][0], ms - q[0][1]]]) for q in qnuml]
+    return qnuml, qnumr, sites
+
+
+class MPODMRG:
+    """The subset of the reference's mpo_dmrg object used on this path."""
+
+    def __init__(self, int1e, int2e, nelec, sz=0.0, Dmax=100, tol=1e-8, crit_vec=1e-8,
+                 occun=None, dtype=0, mpo=None, budget=kernelQt.DEFAULT_BUDGET):
+        self.fhop = mpo if mpo is not None else MPOStore(int1e, int2e)
+        self.nsite = self.fhop.nsite
+        self.sbas = self.fhop.sbas
+        self.nops = self.fhop.nops
+        self.opers = self.fhop.opers
+        self.hpwts = self.fhop.hpwts
+        self.qphys = [qphys() for _ in range(self.nsite)]
+        self.qsectors = {(float(nelec), float(sz)): 1}
+        self.Dmax = Dmax
+        self.crit_e = tol
+        self.crit_vec = crit_vec
+        self.thresh = -1e-14
+        self.dtype = dtype
+        self.budget = budget
+        self.psi0 = None
+        self.nsigma = 0
+        self.sigma_flops = 0
+        self.niters = []
+        self.timing = {'solve': 0.0, 'renorm': 0.0, 'decim': 0.0}
+        K = self.nsite
+        occun = default_occupation(K, nelec, sz) if occun is None else occun
+        self.qnuml, self.qnumr, s0 = mps0occun(K, occun)
+        self.lmps, self.rmps = {}, {}
+        for i in range(K):
+            q = QTensor([False, False, True])
+            q.fromDenseTensor(s0[i], [self.qnuml[i], self.qphys[i], self.qnuml[i + 1]])
+            self.lmps[i] = q
+        # initOps: right boundary + every left environment (util.py:193-214)
+        self.envR = {K: kernelQt.boundary_store(self.nops)}
+        self.envL, _ = kernelQt.genHops(self.fhop, self.lmps, 'L')
+
+
+def sweep_updateSite(dmrg, isite, qred, site, status):
+    """opt.py:313-344 (ncsite=2)."""
+    if status == 'L':
+        dmrg.qnuml[isite + 1] = qred.copy()
+        q = QTensor([False, False, True])
+        q.fromDenseTensor(site, [dmrg.qlst[0], dmrg.qlst[1], qred])
+        dmrg.lmps[isite] = q
+    else:
+        dmrg.qnumr[isite + 1] = qred.copy()
+        q = QTensor([True, False, False])
+        q.fromDenseTensor(site, [qred, dmrg.qlst[-2], dmrg.qlst[-1]])
+        dmrg.rmps[isite + 1] = q
+    return q
+
+
+def sweep_updateBoundary(dmrg, qred, srotR, status):
+    """opt.py:347-377 (ncsite=2)."""
+    K = dmrg.nsite
+    smat = srotR[0] / np.linalg.norm(srotR[0])
+    key = list(dmrg.qsectors.keys())[0]
+    qsym = np.array([key])
+    if status == 'L':
+        q = QTensor([False, False, True])
+        q.fromDenseTensor(smat, [qred, dmrg.qphys[K - 1], qsym])
+        dmrg.lmps[K - 1] = q
+        dmrg.qnuml[K] = qsym
+    else:
+        q = QTensor([True, False, False])
+        q.fromDenseTensor(smat, [qsym, dmrg.qphys[0], qred])
+        dmrg.rmps[0] = q
+        dmrg.qnumr[0] = qsym
+
+
+def sweep(dmrg, status, max_steps=None):
+    """One half sweep (opt.py:31-95); status 'R' runs right-to-left."""
+    K = dmrg.nsite
+    actlst = list(range(K - 2, -1, -1)) if status == 'R' else list(range(0, K - 1))
+    elst, dlst, nmvp = [], [], 0
+    qred = srotR = None
+    for n, isite in enumerate(actlst):
+        if max_steps is not None and n >= max_steps:
+            break
+        flst = [[dmrg.envL[isite - 1], dmrg.envR[isite + 2]]]
+        t0 = time.time()
+        nm, eigs, sigs, dwts, qred, site, srotR = ci_solver(dmrg, isite, 2, actlst, flst, status)
+        t1 = time.time()
+        nmvp += nm
+        elst.append(eigs)
+        dlst.append(dwts)
+        qsite = sweep_updateSite(dmrg, isite, qred, site, status)
+        flstN = [[None]]
+        kernelQt.renormQt_H00(dmrg, isite, 2, flst, flstN, qsite, status)
+        if status == 'L':
+            dmrg.envL[isite] = flstN[0][0]
+        else:
+            dmrg.envR[isite + 1] = flstN[0][0]
+        t2 = time.time()
+        dmrg.timing['solve'] += t1 - t0
+        dmrg.timing['renorm'] += t2 - t1
+    if max_steps is None:
+        sweep_updateBoundary(dmrg, qred, srotR, status)
+    eavs = np.array([e[0] for e in elst])
+    indx = int(np.argmin(eavs))
+    return nmvp, indx, float(eavs[indx]), elst, dlst
+
+
+def run(dmrg, nsweeps, tol=None):
+    """Fixed2 schedule: sweeps 'R' then 'L'; energy = min of both half-sweep minima
+    (class.py:385-430).  Returns the per-sweep energies."""
+    tol = dmrg.crit_e if tol is None else tol
+    eold = 1e3
+    out = []
+    for _ in range(nsweeps):
+        r1 = sweep(dmrg, 'R')
+        r2 = sweep(dmrg, 'L')
+        eav = min(r1[2], r2[2])
+        out.append(eav)
+        if abs(eav - eold) < tol:
+            break
+        eold = eav
+    return out

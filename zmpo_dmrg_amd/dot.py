@@ -1,0 +1,231 @@
+"""Local two-site solver: template, diagonal, Davidson, decimation, guess.
+
+Counterpart of source/mpo_dmrg_dot.py (ci_solver :24-112) and
+source/mpo_dmrg_dotutil.py (setupQlst :81-95, symmetrySpaceInfo :148-189,
+setupQtmpCI :192-203, genHDiag :206-231, finalizeDot :259-285, decimation
+:288-495, genGuess :498-559) for ncsite=2, one process, noise=0.
+
+Sigma, diagonal and the Davidson vector algebra run on the device.  The
+reduced-density-matrix decimation runs on the host with numpy over the QN
+sectors (per-sector eigh, as mpo_dmrg_qparser.rdm_blkdiag); its truncation
+rule is deterministic: weights sorted descending; runs of weights equal to
+1e-10 relative (1e-14 absolute) form tie clusters ordered by (N, Sz) then
+position; keep the first min(Dcut, n); re-sort the kept states by (N, Sz).
+The reference's own rule depends on Python-2 dict order and an unstable
+argsort (mpo_dmrg_qparser.py:220-233, 259) and cannot be reproduced.
+"""
+import numpy as np
+
+from . import device as dv
+from . import kernelQt
+from .mpo import qphys
+from .pdvdson import EigenSolver
+from .qtensor import QTensor, fromQnums
+
+
+def dpt(q1, q2):
+    """Direct-product QNs, second index fastest (mpo_dmrg_qphys.py:63-68)."""
+    q1, q2 = np.asarray(q1), np.asarray(q2)
+    return (q1[:, None, :] + q2[None, :, :]).reshape(-1, q1.shape[1])
+
+
+def truncation_order(sigs, qsyms):
+    n = len(sigs)
+    order = list(np.argsort(-sigs, kind='stable'))
+    out = []
+    i = 0
+    while i < n:
+        j = i + 1
+        w0 = sigs[order[i]]
+        while j < n and abs(sigs[order[j]] - w0) <= max(1e-10 * abs(w0), 1e-14):
+            j += 1
+        cl = order[i:j]
+        cl.sort(key=lambda k: (qsyms[k][0], qsyms[k][1], k))
+        out += cl
+        i = j
+    return np.array(out, dtype=np.int64)
+
+
+def setupQlst(dmrg, isite):
+    """dotutil.py:81-95 (ncsite=2)."""
+    dmrg.qlst = [np.asarray(dmrg.qnuml[isite]), dmrg.qphys[isite], dmrg.qphys[isite + 1],
+                 np.asarray(dmrg.qnumr[isite + 2])]
+
+
+def setupQtmpCI(dmrg, key):
+    """dotutil.py:192-203 (ncsite=2)."""
+    dmrg.qkey = np.array(key, dtype=np.float64)
+    qnums = [dmrg.qlst[0], dmrg.qlst[1], dmrg.qlst[2], dmrg.qkey - dmrg.qlst[3]]
+    dmrg.qtmp = QTensor([False, False, False, True])
+    rank, qsyms, ndims, idlst = fromQnums(qnums)
+    dmrg.qtmp.fromQsyms(rank, qsyms, ndims, ifallocate=False)
+    dmrg.idlstCI = idlst
+
+
+def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
+    """One two-site step (dot.py:24-112).  Returns (nmvps, eigs, sigs, dwts, qred, site, srotR)."""
+    if ncsite != 2:
+        raise NotImplementedError('device solver implements ncsite=2')
+    fL, fR = flst[0][0:2]
+    ldim = int(fL.meta[0].shape[2])
+    rdim = int(fR.meta[0].shape[2])
+    dmrg.dims = (ldim, 16, rdim, ldim * 16 * rdim)
+    setupQlst(dmrg, isite)
+    info = [dmrg, isite, ncsite, flst, status, ifsym]
+    civecs, eigs, nmvps = [], [], 0
+    for key, neig in dmrg.qsectors.items():
+        setupQtmpCI(dmrg, key)
+        prjmap = dmrg.qtmp.prjmap(dmrg.idlstCI)
+        ndim0 = dmrg.qtmp.size_allowed
+        if ndim0 == 0:
+            raise RuntimeError('no basis in the symmetry sector %s' % (key,))
+        diag_h = kernelQt.HDiagQt(info, ndim0, prjmap)
+        prog = kernelQt._sigma_program(dmrg, isite, fL, fR)
+        dtype = getattr(dmrg, 'dtype', 0)
+        solver = EigenSolver()
+        solver.crit_e = dmrg.crit_e
+        solver.crit_vec = dmrg.crit_vec
+        solver.ndim = ndim0
+        solver.neig = neig
+        solver.diag = dv.upload(diag_h, 0)
+        psi_lo = dv.empty(ndim0, dtype) if dtype else None
+        sig_lo = dv.empty(ndim0, dtype) if dtype else None
+
+        def hvec(x, out):
+            if dtype == 0:
+                prog.apply(dmrg.fhop, fL, fR, x, out, 0)
+            else:
+                psi_lo.copy_(x)
+                prog.apply(dmrg.fhop, fL, fR, psi_lo, sig_lo, dtype)
+                out.copy_(sig_lo)
+            dmrg.nsigma += 1
+            dmrg.sigma_flops += prog.flops
+
+        solver.HVec = hvec
+        v0 = None
+        if dmrg.psi0 is not None:
+            v0 = np.ascontiguousarray(dmrg.psi0[:neig][:, prjmap])
+        eig, vec, nmvp = solver.solve_iter(v0)
+        vec_h = dv.download(vec)
+        civec = np.zeros((neig, dmrg.dims[3]))
+        civec[:, prjmap] = vec_h
+        civecs.append(civec)
+        eigs += list(eig)
+        nmvps += nmvp
+        dmrg.niters.append(solver.niter)
+    civecs = np.vstack(civecs)
+    sigs, dwts, qred, site, srotR = decimation(dmrg, civecs, isite, status)
+    genGuess(dmrg, isite, actlst, civecs, srotR, status)
+    return nmvps, np.array(eigs), sigs, dwts, qred, site, srotR
+
+
+def rdm_blkdiag(mats, classes, thresh, Dcut, side):
+    """Per-sector RDM eigh and truncation (mpo_dmrg_qparser.py:101-180).
+
+    ``mats`` are the C matrices (rows = kept side when side == 'L', columns
+    when side == 'R'); the RDM of each sector is formed from its own rows only.
+    """
+    keys, groups = [], {}
+    for idx, val in enumerate(classes):
+        k = (float(val[0]), float(val[1]))
+        if k not in groups:
+            groups[k] = []
+            keys.append(k)
+        groups[k].append(idx)
+    nrows = len(classes)
+    tr = 0.0
+    blocks = []
+    for k in keys:
+        rows = np.array(groups[k])
+        if side == 'L':
+            sub = [m[rows] for m in mats]
+            rho = sum(s @ s.T for s in sub)
+        else:
+            sub = [m[:, rows] for m in mats]
+            rho = sum(s.T @ s for s in sub)
+        tr += np.trace(rho)
+        blocks.append((k, rows, rho))
+    if tr < -1e-12:
+        raise RuntimeError('RDM trace negative')
+    rot_cols, sigs, qsym = [], [], []
+    for k, rows, rho in blocks:
+        if tr < 1e-12:
+            rho = rho + np.identity(len(rows)) / float(nrows)
+        else:
+            rho = rho / tr
+        s2, u = np.linalg.eigh(rho)
+        s2 = np.abs(s2)
+        for c in range(len(rows)):
+            rot_cols.append((rows, u[:, c]))
+        sigs.append(s2)
+        qsym += [k] * len(rows)
+    sigs = np.hstack(sigs)
+    sigs = sigs / np.sum(sigs)
+    indx = truncation_order(sigs, qsym)
+    nres = len(sigs)
+    for i in range(nres):
+        if sigs[indx[i]] < thresh * 1.01:
+            nres = i
+            break
+    nres = max(nres, 1)
+    if Dcut > 0:
+        nres = min(nres, Dcut)
+    indx = indx[:nres]
+    re = sorted(range(nres), key=lambda x: qsym[indx[x]])
+    indx = indx[re]
+    rot = np.zeros((nrows, nres))
+    for c, i in enumerate(indx):
+        rows, vec = rot_cols[i]
+        rot[rows, c] = vec
+    sig_kept = sigs[indx]
+    return 1.0 - np.sum(sig_kept), np.array([qsym[i] for i in indx]), rot, sig_kept
+
+
+def decimation(dmrg, civecs, isite, status):
+    """dotutil.py:288-495 for ncsite=2, noise=0."""
+    ldim, cdim, rdim, _ = dmrg.dims
+    neig = civecs.shape[0]
+    mats = civecs.reshape(neig, ldim * 4, 4 * rdim)
+    if status == 'L':
+        classes = dpt(dmrg.qnuml[isite], dmrg.qphys[isite])
+        dwts, qred, rot, sigs = rdm_blkdiag(mats, classes, dmrg.thresh, dmrg.Dmax, 'L')
+        nres = len(qred)
+        site = rot.reshape(ldim, 4, nres).copy()
+        srotR = np.array([_sparse_matmul(rot.T, m).reshape(nres, 4, rdim) for m in mats])
+    else:
+        classes = dpt(dmrg.qphys[isite + 1], dmrg.qnumr[isite + 2])
+        dwts, qred, rot, sigs = rdm_blkdiag(mats, classes, dmrg.thresh, dmrg.Dmax, 'R')
+        nres = len(qred)
+        site = rot.reshape(4, rdim, nres).transpose(2, 0, 1).copy()
+        srotR = np.array([_sparse_matmul(m, rot).reshape(ldim, 4, nres) for m in mats])
+    return sigs, dwts, qred, site, srotR
+
+
+def genGuess(dmrg, isite, actlst, civecs, srotR, status):
+    """dotutil.py:498-559 (ncsite=2) with block-sparse host products."""
+    if isite == actlst[-1]:
+        dmrg.psi0 = civecs.copy()
+        return
+    out = []
+    for ie in range(civecs.shape[0]):
+        if status == 'R':
+            t = dmrg.lmps[isite - 1].toDenseTensor()                     # (l', 4, l)
+            a = t.reshape(-1, t.shape[2])
+            b = srotR[ie].reshape(t.shape[2], -1)
+        else:
+            t = dmrg.rmps[isite + 2].toDenseTensor()                     # (r, 4, r')
+            a = srotR[ie].reshape(-1, t.shape[0])
+            b = t.reshape(t.shape[0], -1)
+        out.append(_sparse_matmul(a, b).reshape(-1))
+    dmrg.psi0 = np.array(out)
+
+
+def _sparse_matmul(a, b):
+    """a @ b skipping all-zero rows/columns of the block-sparse operands."""
+    rows = np.flatnonzero(np.any(a != 0, axis=1))
+    mid = np.flatnonzero(np.any(a != 0, axis=0) & np.any(b != 0, axis=1))
+    cols = np.flatnonzero(np.any(b != 0, axis=0))
+    out = np.zeros((a.shape[0], b.shape[1]))
+    if len(rows) and len(mid) and len(cols):
+        out[np.ix_(rows, cols)] = a[np.ix_(rows, mid)] @ b[np.ix_(mid, cols)]
+    return out

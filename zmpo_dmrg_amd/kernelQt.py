@@ -1,0 +1,540 @@
+"""Device two-site sigma, diagonal, renormalisation and environment build.
+
+Drop-in counterparts of source/mpo_dmrg_kernelQt.py (HDiagQt :30-127, HVecQt
+:153-319, renormQt_H00 :633-767) and source/mpo_dmrg_initQt.py (genBopsQt
+:49-85, genHopsQt :88-271) with maxslc=1.  Operator-level work is the same
+block-pair contraction as the reference, lowered to libzmpo grouped-GEMM
+launches:
+
+* sigma: 4 launches per op chunk (all ops of the chunk in each launch); the
+  last launch sums hpwts[p] * top4_p over every op of the chunk directly
+  into sigma, one owner per sigma block, so no per-op reduction pass exists;
+* renormalisation: 3 launches per op chunk, new environments written in place
+  into one device buffer per bond;
+* diagonal: 2 launches (L-diagonal x MPO-diagonal, then x R-diagonal).
+
+Ops of one ``case`` (qtensor_opers.py:72-90) share block structure, so one
+symbolic plan per case is instantiated for every op by adding base offsets.
+
+Device layouts (leg storage order inside each block):
+  environments [bra, mpo, ket]; top1 [b, w, n1, n2, r]; top3 [b, wr2, r, u1, u2];
+  every other tensor in the reference's row-major order.
+"""
+import numpy as np
+
+from . import _lib
+from . import device as dv
+from .qtensor import Program, QTensor, gather, run_program
+
+ENV_PERM = (1, 0, 2)
+TOP1_PERM = (1, 0, 2, 3, 4)
+TOP3_PERM = (0, 3, 1, 2, 4)
+DEFAULT_BUDGET = 32 * 2 ** 30     # bytes of intermediates per op chunk
+
+
+def case(porb, isite, ncsite=1):
+    """Operator class relative to the dot (qtensor_opers.py:72-90)."""
+    if porb // 2 < isite:
+        return porb % 2
+    if porb // 2 == isite:
+        return 2 + porb % 2
+    if ncsite == 1 or porb // 2 == isite + 1:
+        return 4 + porb % 2
+    return 6 + porb % 2
+
+
+# ------------------------------------------------------------------ stores
+class EnvStore:
+    """Environments of one bond for every op: ``meta[p]`` (QTensor, ENV_PERM)
+    and one device buffer ``buf`` with op p at ``off[p]`` (replaces the HDF5
+    files lop_site_i / rop_site_i of mpo_dmrg_opt.py:196-238)."""
+
+    def __init__(self, meta, buf, off):
+        self.meta, self.buf, self.off = meta, buf, np.asarray(off, dtype=np.int64)
+        self._buf32 = None
+
+    @property
+    def nops(self):
+        return len(self.meta)
+
+    def buffer(self, dtype=0):
+        if isinstance(self.buf, np.ndarray):
+            self.buf = dv.upload(self.buf, 0)
+        if dtype == 0:
+            return self.buf
+        if self._buf32 is None:
+            self._buf32 = dv.empty(self.buf.numel(), 1)
+            _lib.check(_lib.lib().zmpo_convert(1, dv.ptr(self._buf32), 0, dv.ptr(self.buf),
+                                               self.buf.numel(), dv.stream()))
+        return self._buf32
+
+    def op_tensor(self, p):
+        """QTensor of op p with a host copy of its values (tests, export)."""
+        m = self.meta[p]
+        qt = m.template()
+        qt.idlst = m.idlst
+        n = m.size_allowed
+        b = self.buf
+        seg = b[self.off[p]:self.off[p] + n]
+        qt.value = (seg.copy() if isinstance(b, np.ndarray) else dv.download(seg)) if n \
+            else np.zeros(0)
+        return qt
+
+
+def boundary_store(nops, nqnum=2):
+    """1x1x1 boundary operators with zero QN (initQt.py:49-85)."""
+    qt = QTensor([True, False, True], ENV_PERM)
+    z = np.zeros((1, nqnum))
+    qt.fromQsyms(3, [z, z, z], [np.ones(1, np.int64)] * 3, ifallocate=False,
+                 idlst=[[[0]], [[0]], [[0]]])
+    return EnvStore([qt] * nops, np.ones(nops), np.arange(nops))
+
+
+def _instantiate(lowered, a_base, b_base, c_base, scale=1.0, p_shift=0):
+    """Copy a case plan for one op with operand base offsets."""
+    t = lowered['tasks'].copy()
+    p = lowered['pairs'].copy()
+    p['a_off'] += a_base
+    p['b_off'] += b_base
+    p['scale'] = scale
+    t['c_off'] += c_base
+    t['p_begin'] += p_shift
+    t['p_end'] += p_shift
+    return t, p
+
+
+def _concat(parts):
+    ts = [t for t, _ in parts]
+    ps = [p for _, p in parts]
+    if not ts:
+        return np.zeros(0, _lib.TASK_DTYPE), np.zeros(0, _lib.PAIR_DTYPE)
+    return np.concatenate(ts), np.concatenate(ps)
+
+
+def _merge_shared(parts, nout):
+    """Group pairs of several ops that write the same output blocks.
+
+    Each part's pairs are in task order; the stable sort keeps, inside every
+    output block, the ops in chunk order (fixed summation order)."""
+    rows, pairs = [], []
+    for t, p in parts:
+        cnt = (t['p_end'] - t['p_begin']).astype(np.int64)
+        rows.append(np.repeat(np.arange(len(t)), cnt))
+        pairs.append(p)
+    rows = np.concatenate(rows)
+    pairs = np.concatenate(pairs)
+    order = np.argsort(rows, kind='stable')
+    pairs = pairs[order]
+    counts = np.bincount(rows, minlength=nout)
+    tasks = parts[0][0].copy()
+    tasks['p_begin'] = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    tasks['p_end'] = tasks['p_begin'] + counts
+    return tasks, pairs
+
+
+def _chunks(sizes, budget):
+    """Split ops into consecutive chunks whose intermediate bytes fit the budget."""
+    out, cur, acc = [], [], 0
+    for p, s in enumerate(sizes):
+        if cur and acc + s > budget:
+            out.append(cur)
+            cur, acc = [], 0
+        cur.append(p)
+        acc += s
+    if cur:
+        out.append(cur)
+    return out
+
+
+# ------------------------------------------------------------------ sigma
+class SigmaProgram:
+    """Compiled two-site sigma for one (site, psi template): per op chunk,
+    four grouped-GEMM programs and the intermediate offsets."""
+
+    def __init__(self, mpo, isite, qtmp, qkey, lenv, renv, hpwts, budget=DEFAULT_BUDGET):
+        self.isite = isite
+        self.ndim0 = qtmp.size_allowed
+        jsite = isite + 1
+        nops = mpo.nops
+        plans = {}
+        self.flops_stage = np.zeros(4, dtype=np.int64)
+        for p in range(nops):
+            c = case(mpo.opers[p], isite, 2)
+            if c in plans:
+                continue
+            lop = lenv.meta[p]
+            cop = mpo.sites[isite][p]
+            wop = mpo.sites[jsite][p]
+            rop = renv.meta[p]
+            t1 = QTensor()
+            t1.tensordotSYM(lop, qtmp, ([2], [0]), perm=TOP1_PERM)
+            t2 = QTensor()
+            t2.tensordotSYM(t1, cop, ([0, 2], [0, 3]))
+            t3 = QTensor()
+            t3.tensordotSYM(t2, wop, ([1, 3], [3, 0]), perm=TOP3_PERM)
+            rf = rop.template()
+            rf.qsyms = [-rop.qsyms[0], qkey - rop.qsyms[1], qkey - rop.qsyms[2]]
+            t4 = QTensor()
+            t4.tensordotSYM(t3, rf, ([1, 3], [2, 0]))
+            if t4.nblks_allowed and not (np.array_equal(t4.idx_allowed, qtmp.idx_allowed)
+                                         and np.array_equal(t4.blk_dims, qtmp.blk_dims)):
+                raise RuntimeError('sigma output structure differs from the psi template')
+            plans[c] = [t.plan.lowered() for t in (t1, t2, t3, t4)] + [t1, t2, t3]
+        self.plans = plans
+        self.case_of = [case(mpo.opers[p], isite, 2) for p in range(nops)]
+        sz = [sum(plans[c][k].size_allowed for k in (4, 5, 6)) * 8 for c in self.case_of]
+        self.chunks = _chunks(sz, budget)
+        self.stages = []
+        self.inter = [0, 0, 0]
+        for ch in self.chunks:
+            offs = [0, 0, 0]
+            parts = [[], [], [], []]
+            for p in ch:
+                pl = plans[self.case_of[p]]
+                sizes = [pl[4].size_allowed, pl[5].size_allowed, pl[6].size_allowed]
+                if pl[4].nblks_allowed == 0 or pl[3]['pairs'].size == 0:
+                    continue
+                lo = lenv.off[p]
+                wo, vo = mpo.off[isite][p], mpo.off[jsite][p]
+                ro = renv.off[p]
+                shift = [sum(len(x[1]) for x in parts[k]) for k in range(4)]
+                parts[0].append(_instantiate(pl[0], lo, 0, offs[0], 1.0, shift[0]))
+                parts[1].append(_instantiate(pl[1], offs[0], wo, offs[1], 1.0, shift[1]))
+                parts[2].append(_instantiate(pl[2], offs[1], vo, offs[2], 1.0, shift[2]))
+                parts[3].append(_instantiate(pl[3], offs[2], ro, 0, float(hpwts[p]), 0))
+                for k in range(3):
+                    offs[k] += sizes[k]
+            progs = []
+            for k in range(3):
+                t, pr = _concat(parts[k])
+                progs.append(Program(t, pr))
+            t, pr = _merge_shared(parts[3], qtmp.nblks_allowed) if parts[3] else _concat([])
+            progs.append(Program(t, pr))
+            for k in range(4):
+                self.flops_stage[k] += progs[k].flops
+            self.stages.append(progs)
+            self.inter = [max(a, b) for a, b in zip(self.inter, offs)]
+        self.flops = int(self.flops_stage.sum())
+        self._buffers = {}
+
+    def buffers(self, dtype):
+        if dtype not in self._buffers:
+            self._buffers[dtype] = [dv.empty(n, dtype) for n in self.inter]
+        return self._buffers[dtype]
+
+    def apply(self, mpo, lenv, renv, psi, sigma, dtype=0):
+        """sigma = H psi (device arrays of length ndim0, same dtype)."""
+        t1, t2, t3 = self.buffers(dtype)
+        L, R = lenv.buffer(dtype), renv.buffer(dtype)
+        Wi, Wj = mpo.buffer(self.isite, dtype), mpo.buffer(self.isite + 1, dtype)
+        if not self.stages or all(st[3].ntasks == 0 for st in self.stages):
+            sigma[:] = 0
+            return sigma
+        first = True
+        for st in self.stages:
+            run_program(st[0], L, psi, t1, dtype)
+            run_program(st[1], t1, Wi, t2, dtype)
+            run_program(st[2], t2, Wj, t3, dtype)
+            if st[3].ntasks:
+                run_program(st[3], t3, R, sigma, dtype, beta=0 if first else 1)
+                first = False
+        return sigma
+
+
+# ------------------------------------------------------------------ diagonal
+def _zero_sector(qsyms0):
+    z = np.flatnonzero(np.all(np.abs(qsyms0) < 1e-10, axis=1))
+    return int(z[0]) if len(z) else -1
+
+
+def _diag_gather_index(meta):
+    """Indices of env[b, w, b] for w in the zero MPO sector, as [w][b'] with b'
+    running over the bra sectors in sector order.  Returns (idx, zdim, bdim)."""
+    z = _zero_sector(meta.qsyms[0])
+    bdim = int(meta.ndims[1].sum())
+    if z < 0:
+        return None, 0, bdim
+    zd = int(meta.ndims[0][z])
+    soff = np.concatenate([[0], np.cumsum(meta.ndims[1])[:-1]])
+    idx = -np.ones((zd, bdim), dtype=np.int64)
+    st = meta.blk_strides()
+    for j in range(meta.nblks_allowed):
+        w_s, b_s, k_s = meta.blk_isyms[j]
+        if w_s != z or b_s != k_s:
+            continue
+        d = int(meta.ndims[1][b_s])
+        bl = np.arange(d)
+        wl = np.arange(zd)
+        off = (meta.blk_off[j] + bl[None, :] * st[j, 1] + wl[:, None] * st[j, 0]
+               + bl[None, :] * st[j, 2])
+        idx[:, soff[b_s]:soff[b_s] + d] = off
+    return idx, zd, bdim
+
+
+class DiagProgram:
+    """HDiagQt: diag[l,n1,n2,r] = sum_p hp sum_{w,wr,wr2} Ld_p[w,l] Wi_p[w,wr,n1,n1]
+    Wj_p[wr,wr2,n2,n2] Rd_p[wr2,r], on the psi template's allowed blocks."""
+
+    def __init__(self, mpo, isite, qtmp, lenv, renv, hpwts):
+        jsite = isite + 1
+        nops = mpo.nops
+        self.ndim0 = qtmp.size_allowed
+        lgi, rgi, pmats = [], [], []
+        lbase = rbase = xbase = 0
+        ldim = int(qtmp.ndims[0].sum())
+        rdim = int(qtmp.ndims[3].sum())
+        loff = np.concatenate([[0], np.cumsum(qtmp.ndims[0])[:-1]])
+        roff = np.concatenate([[0], np.cumsum(qtmp.ndims[3])[:-1]])
+        h1_parts, h2_rows = [], []
+        ops_info = []
+        pbase = 0
+        for p in range(nops):
+            li, z1, ld = _diag_gather_index(lenv.meta[p])
+            ri, z3, rd = _diag_gather_index(renv.meta[p])
+            if li is None or ri is None:
+                continue
+            assert ld == ldim and rd == rdim
+            cop, wop = mpo.sites[isite][p], mpo.sites[jsite][p]
+            cd = cop.toDenseTensor()
+            wd = wop.toDenseTensor()
+            zc = _zero_sector(cop.qsyms[0])
+            zw = _zero_sector(wop.qsyms[1])
+            if zc < 0 or zw < 0:
+                continue
+            ci = cop.idlst[0][zc]
+            wi = wop.idlst[1][zw]
+            cdz = np.einsum('abnn->abn', cd[ci])                 # [w, wr, n]
+            wdz = np.einsum('abnn->abn', wd[:, wi])              # [wr, wr2, n]
+            P = np.einsum('awn,wbm->nmab', cdz, wdz).reshape(16, len(ci), len(wi))
+            if not np.any(P):
+                continue
+            lgi.append(li.reshape(-1) + np.where(li.reshape(-1) >= 0, lenv.off[p], 0))
+            rgi.append(ri.reshape(-1) + np.where(ri.reshape(-1) >= 0, renv.off[p], 0))
+            pmats.append(P.reshape(-1))
+            ops_info.append((p, z1, z3, lbase, rbase, pbase, xbase))
+            lbase += z1 * ldim
+            rbase += z3 * rdim
+            pbase += P.size
+            xbase += 16 * ldim * z3
+        self.nact = len(ops_info)
+        if not self.nact:
+            self.h1 = self.h2 = None
+            return
+        self.lidx = dv.upload(np.concatenate(lgi))
+        self.ridx = dv.upload(np.concatenate(rgi))
+        self.pbuf = dv.upload(np.concatenate(pmats), 0)
+        self.nl, self.nr, self.nx = lbase, rbase, xbase
+        # H1: X_p[nn][l, wr2] = sum_w Ld_p[w, l] P_p[nn][w, wr2]
+        t1 = np.zeros(16 * self.nact, dtype=_lib.TASK_DTYPE)
+        p1 = np.zeros(16 * self.nact, dtype=_lib.PAIR_DTYPE)
+        k = 0
+        for (p, z1, z3, lb, rb, pb, xb) in ops_info:
+            for nn in range(16):
+                t1[k] = (xb + nn * ldim * z3, 0, ldim, z3, ldim, 0, z3, 1, k, k + 1, 0, 0)
+                p1[k] = (lb, pb + nn * z1 * z3, 1.0, z1, ldim, 0, 1, ldim, z3, 0, 1, z3, 0)
+                k += 1
+        self.h1 = Program(t1, p1)
+        # H2: diag block j = sum_p hp X_p[n1n2][l in l_s, :] Rd_p[:, r in r_s]
+        nb = qtmp.nblks_allowed
+        iso = qtmp.blk_isyms
+        t2 = np.zeros(nb, dtype=_lib.TASK_DTYPE)
+        p2 = np.zeros(nb * self.nact, dtype=_lib.PAIR_DTYPE)
+        k = 0
+        for j in range(nb):
+            ls, n1, n2, rs = iso[j]
+            dl, dr = int(qtmp.ndims[0][ls]), int(qtmp.ndims[3][rs])
+            nn = int(n1) * 4 + int(n2)
+            beg = k
+            for (p, z1, z3, lb, rb, pb, xb) in ops_info:
+                p2[k] = (xb + nn * ldim * z3 + loff[ls] * z3, rb + roff[rs], float(hpwts[p]),
+                         z3, dl, 0, z3, 1, dr, 0, 1, rdim, 0)
+                k += 1
+            t2[j] = (qtmp.blk_off[j], 0, dl, dr, dl, 0, dr, 1, beg, k, 0, 0)
+        self.h2 = Program(t2, p2)
+        self.flops = self.h1.flops + self.h2.flops
+
+    def apply(self, lenv, renv, out):
+        if not self.nact:
+            out[:] = 0
+            return out
+        ld = gather(dv.empty(self.nl, 0), lenv.buffer(0), self.lidx)
+        rd = gather(dv.empty(self.nr, 0), renv.buffer(0), self.ridx)
+        x = dv.empty(self.nx, 0)
+        run_program(self.h1, ld, self.pbuf, x, 0)
+        run_program(self.h2, x, rd, out, 0)
+        return out
+
+
+# ------------------------------------------------------------------ renorm
+class RenormProgram:
+    """Three-stage renormalisation of every op for one site (kernelQt.py:633-767)."""
+
+    def __init__(self, mpo, site_idx, qsite, env_in, status, budget=DEFAULT_BUDGET):
+        nops = mpo.nops
+        self.site_idx = site_idx
+        self.status = status
+        plans = {}
+        case_of = [case(mpo.opers[p], site_idx) for p in range(nops)]
+        for p in range(nops):
+            c = case_of[p]
+            if c in plans:
+                continue
+            cop = mpo.sites[site_idx][p]
+            op_in = env_in.meta[p]
+            t1, t2, t3 = QTensor(), QTensor(), QTensor()
+            if status == 'L':
+                t1.tensordotSYM(qsite, op_in, ([0], [1]))
+                t2.tensordotSYM(cop, t1, ([0, 2], [2, 0]))
+                t3.tensordotSYM(t2, qsite, ([1, 3], [1, 0]), perm=ENV_PERM)
+                t3.idlst = [cop.idlst[1], qsite.idlst[2], qsite.idlst[2]]
+            else:
+                cf = cop.template()
+                cf.qsyms = [-cop.qsyms[0], -cop.qsyms[1], cop.qsyms[2], cop.qsyms[3]]
+                cf.status = cop.status.copy()
+                cf.status[0] = not cf.status[0]
+                cf.status[1] = not cf.status[1]
+                cf.fromQsyms(4, cf.qsyms, cf.ndims, ifallocate=False, idlst=cop.idlst)
+                t1.tensordotSYM(qsite, op_in, ([2], [1]))
+                t2.tensordotSYM(cf, t1, ([1, 2], [2, 1]))
+                t3.tensordotSYM(t2, qsite, ([1, 3], [1, 2]), perm=ENV_PERM)
+                t3.idlst = [cop.idlst[0], qsite.idlst[0], qsite.idlst[0]]
+            plans[c] = [t.plan.lowered() for t in (t1, t2, t3)] + [t1, t2, t3]
+        self.plans = plans
+        self.meta = [plans[c][5] for c in case_of]
+        sizes = np.array([m.size_allowed for m in self.meta], dtype=np.int64)
+        self.out_off = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        self.out_size = int(sizes.sum())
+        isz = [(plans[c][3].size_allowed + plans[c][4].size_allowed) * 8 for c in case_of]
+        self.chunks = _chunks(isz, budget)
+        self.stages = []
+        self.inter = [0, 0]
+        self.flops = 0
+        for ch in self.chunks:
+            offs = [0, 0]
+            parts = [[], [], []]
+            for p in ch:
+                pl = plans[case_of[p]]
+                shift = [sum(len(x[1]) for x in parts[k]) for k in range(3)]
+                if status == 'L':
+                    parts[0].append(_instantiate(pl[0], 0, env_in.off[p], offs[0], 1.0, shift[0]))
+                    parts[1].append(_instantiate(pl[1], mpo.off[site_idx][p], offs[0], offs[1],
+                                                 1.0, shift[1]))
+                else:
+                    parts[0].append(_instantiate(pl[0], 0, env_in.off[p], offs[0], 1.0, shift[0]))
+                    parts[1].append(_instantiate(pl[1], mpo.off[site_idx][p], offs[0], offs[1],
+                                                 1.0, shift[1]))
+                parts[2].append(_instantiate(pl[2], offs[1], 0, self.out_off[p], 1.0, shift[2]))
+                offs[0] += pl[3].size_allowed
+                offs[1] += pl[4].size_allowed
+            progs = [Program(*_concat(parts[k])) for k in range(3)]
+            self.flops += sum(pg.flops for pg in progs)
+            self.stages.append(progs)
+            self.inter = [max(a, b) for a, b in zip(self.inter, offs)]
+
+    def apply(self, mpo, site_values, env_in, dtype=0):
+        """Return the new EnvStore (values on the device)."""
+        t1 = dv.empty(self.inter[0], dtype)
+        t2 = dv.empty(self.inter[1], dtype)
+        out = dv.zeros(self.out_size, dtype)
+        W = mpo.buffer(self.site_idx, dtype)
+        E = env_in.buffer(dtype)
+        for st in self.stages:
+            run_program(st[0], site_values, E, t1, dtype)
+            run_program(st[1], W, t1, t2, dtype)
+            run_program(st[2], t2, site_values, out, dtype)
+        if dtype != 0:
+            o64 = dv.empty(self.out_size, 0)
+            _lib.check(_lib.lib().zmpo_convert(0, dv.ptr(o64), dtype, dv.ptr(out),
+                                               self.out_size, dv.stream()))
+            out = o64
+        return EnvStore(self.meta, out, self.out_off)
+
+
+def renorm(mpo, site_idx, qsite, env_in, status, dtype=0):
+    """One renormalisation step; qsite.value may be host or device."""
+    prog = RenormProgram(mpo, site_idx, qsite, env_in, status)
+    v = qsite.value
+    v = dv.upload(v, dtype) if isinstance(v, np.ndarray) else (v if dtype == 0 else v.float())
+    return prog.apply(mpo, v, env_in, dtype), prog
+
+
+def genHops(mpo, sites, status, nqnum=2):
+    """All environments from an MPS (initQt.py:88-271).  Returns (envs, exphop)."""
+    K = mpo.nsite
+    envs = {}
+    if status == 'L':
+        envs[-1] = boundary_store(mpo.nops, nqnum)
+        for i in range(K):
+            envs[i], _ = renorm(mpo, i, sites[i], envs[i - 1], 'L')
+        last = envs[K - 1]
+    else:
+        envs[K] = boundary_store(mpo.nops, nqnum)
+        for i in range(K - 1, -1, -1):
+            envs[i], _ = renorm(mpo, i, sites[i], envs[i + 1], 'R')
+        last = envs[0]
+    vals = last.buf if isinstance(last.buf, np.ndarray) else dv.download(last.buf)
+    exphop = np.array([vals[last.off[p]] if last.meta[p].size_allowed else 0.0
+                       for p in range(mpo.nops)])
+    return envs, exphop
+
+
+# ------------------------------------------------------------------ drop-in API
+def _info_parts(info):
+    dmrg, isite, ncsite, flst, status, ifsym = info
+    if ncsite != 2:
+        raise NotImplementedError('the device path implements the two-site (ncsite=2) kernels')
+    return dmrg, isite, flst
+
+
+def _sigma_program(dmrg, isite, fL, fR):
+    key = (isite, id(dmrg.qtmp), id(fL), id(fR))
+    cache = getattr(dmrg, '_sigma_cache', None)
+    if cache is None or cache[0] != key:
+        prog = SigmaProgram(dmrg.fhop, isite, dmrg.qtmp, dmrg.qkey, fL, fR, dmrg.hpwts,
+                            getattr(dmrg, 'budget', DEFAULT_BUDGET))
+        dmrg._sigma_cache = (key, prog)
+    return dmrg._sigma_cache[1]
+
+
+def HVecQt(vec0, info, ndim0, prjmap, ifile=0, ioff=0):
+    """sigma = sum_p hpwts[p] H_p psi for the two-site dot (kernelQt.py:153-319).
+
+    vec0: float64[ndim0] in qtmp block order (numpy or device tensor); the
+    result has the same kind.  flst[ifile][ioff:ioff+2] are the L/R EnvStores.
+    """
+    dmrg, isite, flst = _info_parts(info)
+    fL, fR = flst[ifile][ioff:ioff + 2]
+    prog = _sigma_program(dmrg, isite, fL, fR)
+    dtype = getattr(dmrg, 'dtype', 0)
+    host = isinstance(vec0, np.ndarray)
+    psi = dv.upload(vec0, dtype) if host else vec0
+    sig = dv.empty(ndim0, dtype)
+    prog.apply(dmrg.fhop, fL, fR, psi, sig, dtype)
+    return dv.download(sig).astype(np.float64) if host else sig
+
+
+def HDiagQt(info, ndim0, prjmap):
+    """Diagonal of H_eff in qtmp order (kernelQt.py:30-127); returns float64 numpy."""
+    dmrg, isite, flst = _info_parts(info)
+    fL, fR = flst[0][0:2]
+    prog = DiagProgram(dmrg.fhop, isite, dmrg.qtmp, fL, fR, dmrg.hpwts)
+    out = dv.zeros(ndim0, 0)
+    prog.apply(fL, fR, out)
+    return dv.download(out)
+
+
+def renormQt_H00(dmrg, isite, ncsite, flst, flstN, site, status):
+    """New environments for every op (kernelQt.py:633-767).
+
+    The result EnvStore is stored into flstN[0][0] (the reference writes the
+    HDF5 file opened there).  Returns 0.
+    """
+    if status == 'L':
+        env_in = flst[0][0]
+        idx = isite
+    else:
+        env_in = flst[0][1]
+        idx = isite + ncsite - 1
+    new, _ = renorm(dmrg.fhop, idx, site, env_in, status, getattr(dmrg, 'dtype', 0))
+    flstN[0][0] = new
+    return 0

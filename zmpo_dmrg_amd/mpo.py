@@ -1,0 +1,192 @@
+"""MPO site tensors H_p[i] = a_p^dagger V_p and the device-resident MPO store.
+
+Dense factors follow source/mpo_dmrg_opers.py:91-219 (genWfacSpin,
+genHfacSpin, linkTwoOpers); quantum numbers and the block-sparse form follow
+source/qtensor/qtensor_opers.py:95-193 and :358-408 (genHRfacSpatialQt with
+maxslc=1).  The store replaces the HDF5 file ``hop`` written by
+source/mpo_dmrg_io.py:168-206: every site's ops are concatenated into one
+device buffer so a single grouped-GEMM launch can address all of them.
+"""
+import numpy as np
+
+from . import device as dv
+from .qtensor import QTensor
+
+_S = {
+    'sgn': np.array([[1., 0.], [0., -1.]]),
+    'idn': np.eye(2),
+    'idnt': np.array([[1., 0.], [0., -1.]]),
+    'cre': np.array([[0., 0.], [1., 0.]]),
+    'cret': np.array([[0., 0.], [1., 0.]]),
+    'ann': np.array([[0., 1.], [0., 0.]]),
+    'annt': np.array([[0., -1.], [0., 0.]]),
+    'nii': np.array([[0., 0.], [0., 1.]]),
+    'niit': np.array([[0., 0.], [0., -1.]]),
+}
+SPINS = (0.5, -0.5)
+
+
+def wfac_spin(nsite, isite, hq, vqrs):
+    """V_p factor on one spin orbital (mpo_dmrg_opers.py:91-194), vectorised.
+
+    Bond layout: [0] | q1(right sites) | q2(right sites) | q3(left sites) |
+    q4(left sites) | last; the first/last site keep row 0 / the last column.
+    """
+    S = _S
+    nb = 2 * (nsite + 1)
+    diml, dimr = isite, nsite - isite - 1
+    rs = np.arange(isite + 1, nsite)
+    ls = np.arange(diml)
+    last = nb - 1
+    w = np.zeros((nb, nb, 2, 2))
+    w[0, 0] = S['idnt']
+    w[0, 1 + dimr:1 + 2 * dimr] = vqrs[isite, isite, rs][:, None, None] * S['niit']
+    w[0, 1 + 2 * dimr + diml] = -S['ann']
+    w[0, 2 + 2 * dimr + 2 * diml] = S['cre']
+    w[0, last] = hq[isite] * S['ann']
+    w[1, last] = S['cre']
+    w[2 + np.arange(dimr), 1 + np.arange(dimr)] = S['idnt']
+    w[2 + dimr, last] = S['ann']
+    w[3 + dimr + np.arange(dimr), 1 + dimr + np.arange(dimr)] = S['idnt']
+    off = 3 + 2 * dimr
+    if diml and dimr:
+        a = vqrs[np.ix_(rs, ls, [isite])][:, :, 0].T            # [l, r] = v[r, l, i]
+        b = vqrs[np.ix_([isite], ls, rs)][0]                     # [l, r] = v[i, l, r]
+        c = vqrs[np.ix_(ls, [isite], rs)][:, 0, :]               # [l, r] = v[l, i, r]
+        w[off:off + diml, 1:1 + dimr] = -a[:, :, None, None] * S['annt']
+        w[off:off + diml, 1 + dimr:1 + 2 * dimr] = b[:, :, None, None] * S['cret']
+        off4 = off + diml
+        w[off4:off4 + diml, 1 + dimr:1 + 2 * dimr] = c[:, :, None, None] * S['annt']
+    if diml:
+        w[off + ls, 1 + 2 * dimr + ls] = S['idn']
+        w[off:off + diml, last] = vqrs[isite, ls, isite][:, None, None] * S['nii']
+        off4 = off + diml
+        w[off4 + ls, 2 + 2 * dimr + diml + ls] = S['idn']
+    w[last, last] = S['idn']
+    if isite == 0:
+        w = w[:1]
+    if isite == nsite - 1:
+        w = w[:, last:]
+    return w
+
+
+def hfac_spin(p, nsite, isite, hq, vqrs):
+    """(mpo_dmrg_opers.py:56-66, 197-203): elem(a_p^dagger) applied on V_p."""
+    e = _S['sgn'] if isite < p else (_S['cre'] if isite == p else _S['idn'])
+    return np.einsum('ij,abjk->abik', e, wfac_spin(nsite, isite, hq, vqrs))
+
+
+def link(op1, op2):
+    """W1[a,b,i,j] W2[b,c,k,l] -> W[a,c,(ik),(jl)] (mpo_dmrg_opers.py:43-49)."""
+    t = np.tensordot(op1, op2, axes=([1], [0])).transpose(0, 3, 1, 4, 2, 5)
+    s = t.shape
+    return t.reshape(s[0], s[1], s[2] * s[3], s[4] * s[5])
+
+
+def qphys():
+    """|0>, |b>, |a>, |ab> as (N, Sz) (qtensor_opers.py:95-109)."""
+    return np.array([[0., 0.], [1., -0.5], [1., 0.5], [2., 0.]])
+
+
+def _elem_q(p, isite):
+    spin = 0.5 if p % 2 == 0 else -0.5
+    if isite < p:
+        return np.array([0., 0.]), np.array([0., 0.])
+    if isite == p:
+        return np.array([0., 0.]), np.array([1., spin])
+    return np.array([1., spin]), np.array([1., spin])
+
+
+def _right_q(nsite, isite, sz):
+    q = [[0., 0.]]
+    q += [[-2., -sz - SPINS[i % 2]] for i in range(isite + 1, nsite)]
+    q += [[0., -sz + SPINS[i % 2]] for i in range(isite + 1, nsite)]
+    q += [[-1., -SPINS[i % 2]] for i in range(isite + 1)]
+    q += [[1., SPINS[i % 2]] for i in range(isite + 1)]
+    q += [[-1., -sz]]
+    return np.array(q)
+
+
+def _w_q(nsite, isite, isz):
+    sz = SPINS[isz]
+    ql = np.array([[0., 0.]]) if isite == 0 else _right_q(nsite, isite - 1, sz)
+    qr = np.array([[-1., -sz]]) if isite == nsite - 1 else _right_q(nsite, isite, sz)
+    return ql, qr
+
+
+_KEYS = {}
+
+
+def _qkey(row):
+    t = np.asarray(row, dtype=np.float64).tobytes()   # keeps -0.0 apart from 0.0
+    k = _KEYS.get(t)
+    if k is None:
+        k = _KEYS[t] = str(np.asarray(row))
+    return k
+
+
+def _sort_key_order(q):
+    """Stable order by the printed QN row (qtensor_opers.py:56-58)."""
+    keys = [_qkey(r) for r in q]
+    return np.array(sorted(range(len(q)), key=lambda k: keys[k]))
+
+
+def site_tensor(p, nsite, isite, int1e, int2e):
+    """Block-sparse H_p on spatial site isite, legs [wl, wr, u, d], status [F,T,F,T]."""
+    hq, vqrs = int1e[p], int2e[p]
+    w = link(hfac_spin(p, nsite, 2 * isite, hq, vqrs), hfac_spin(p, nsite, 2 * isite + 1, hq, vqrs))
+    isz = p % 2
+    el, _ = _elem_q(p, 2 * isite)
+    _, er = _elem_q(p, 2 * isite + 1)
+    wl, _ = _w_q(nsite, 2 * isite, isz)
+    _, wr = _w_q(nsite, 2 * isite + 1, isz)
+    ql = wl + el[None, :]
+    qr = wr + er[None, :]
+    il, ir = _sort_key_order(ql), _sort_key_order(qr)
+    w = w[np.ix_(il, ir)]
+    qt = QTensor([False, True, False, True])
+    qp = qphys()
+    qt.fromDenseTensor(w, [ql[il], qr[ir], qp, qp], ifcollect=[1, 1, 0, 0])
+    return qt
+
+
+class MPOStore:
+    """All H_p[i] for i < K on the device ("hop" file of io.py:168-206).
+
+    ``sites[i][p]`` is the host QTensor (metadata + values); ``buf[i]`` the
+    device buffer with every op of site i concatenated at ``off[i][p]``.
+    """
+
+    def __init__(self, int1e, int2e, nops=None, upload=True):
+        self.sbas = int1e.shape[0]
+        self.nsite = self.sbas // 2
+        self.nops = self.sbas if nops is None else nops
+        self.opers = list(range(self.nops))
+        self.hpwts = np.ones(self.nops)
+        self.sites = []
+        self.off = []
+        self.host = []
+        for i in range(self.nsite):
+            ops = [site_tensor(p, self.sbas, i, int1e, int2e) for p in self.opers]
+            sizes = np.array([q.size_allowed for q in ops], dtype=np.int64)
+            offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+            self.sites.append(ops)
+            self.off.append(offs)
+            self.host.append(np.concatenate([q.value for q in ops]))
+        self.buf = [None] * self.nsite
+        self._buf32 = {}
+        if upload:
+            self.to_device()
+
+    def to_device(self):
+        self.buf = [dv.upload(h, 0) for h in self.host]
+        return self
+
+    def buffer(self, i, dtype=0):
+        if dtype == 0:
+            if self.buf[i] is None:
+                self.buf[i] = dv.upload(self.host[i], 0)
+            return self.buf[i]
+        if i not in self._buf32:
+            self._buf32[i] = dv.upload(self.host[i], 1)
+        return self._buf32[i]

@@ -1,0 +1,199 @@
+"""Device Davidson eigensolver (counterpart of source/tools/pdvdson.py:22-390).
+
+Basis and sigma vectors live on the device in FP64; every inner product,
+linear combination and the |diag-e|+shift preconditioner are libzmpo kernels
+with a fixed reduction order.  Only the (<= 2*nfac*neig)-dimensional
+Rayleigh-Ritz matrix and the subspace-collapse coefficients are solved on the
+host (numpy eigh of a <= 24x24 matrix), as the reference does on rank 0.
+"""
+import numpy as np
+
+from . import _lib
+from . import device as dv
+
+
+class DavidsonError(RuntimeError):
+    pass
+
+
+class _VecOps:
+    def __init__(self, n):
+        self.n = n
+        self.work = dv.empty(_lib.lib().zmpo_dots_workspace(64), 0)
+        self.out = dv.empty(64, 0)
+
+    def dots(self, X, nx, y):
+        """[X_j . y for j < nx] as a host array (X rows of a 2-D device tensor)."""
+        if nx == 0:
+            return np.zeros(0)
+        _lib.check(_lib.lib().zmpo_dots(0, dv.ptr(X), X.stride(0), nx, dv.ptr(y), self.n,
+                                        dv.ptr(self.out), dv.ptr(self.work), dv.stream()))
+        return dv.download(self.out[:nx]).copy()
+
+    def lincomb(self, out, X, coef, alpha=1.0, beta=0.0):
+        nx = len(coef)
+        if nx == 0:
+            if beta == 0.0:
+                out.zero_()
+            elif beta != 1.0:
+                out.mul_(beta)
+            return out
+        c = dv.upload(np.asarray(coef, dtype=np.float64), 0)
+        _lib.check(_lib.lib().zmpo_lincomb(0, dv.ptr(out), dv.ptr(X), X.stride(0), nx, dv.ptr(c),
+                                           float(alpha), float(beta), self.n, dv.stream()))
+        return out
+
+    def norm(self, x):
+        return float(np.sqrt(max(self.dots(x.view(1, -1), 1, x)[0], 0.0)))
+
+    def precond(self, r, diag, e, shift):
+        _lib.check(_lib.lib().zmpo_precond(0, dv.ptr(r), dv.ptr(diag), float(e), float(shift),
+                                           self.n, dv.stream()))
+
+
+def _host_ortho(vbas, rbas, crit_indp):
+    """Subspace-coefficient version of dvdson_ortho (pdvdson.py:350-390)."""
+    rbas = rbas.copy()
+    v2 = np.zeros_like(rbas)
+    nindp = 0
+    for _ in range(5):
+        rbas = rbas - (rbas @ vbas.T) @ vbas
+    for i in range(rbas.shape[0]):
+        rv = rbas[i].copy()
+        rii = np.linalg.norm(rv)
+        if rii <= crit_indp:
+            continue
+        rv = rv / rii
+        rv = rv / np.linalg.norm(rv)
+        v2[nindp] = rv
+        nindp += 1
+        for _ in range(5):
+            rbas[i:] -= (rbas[i:] @ vbas.T) @ vbas
+            rbas[i:] -= (rbas[i:] @ v2[:nindp].T) @ v2[:nindp]
+    return nindp, v2[:nindp].copy()
+
+
+class EigenSolver:
+    """pdvdson.eigenSolver restricted to iop=4, real, one process."""
+
+    def __init__(self):
+        self.maxcycle = 20
+        self.crit_e = 1e-8
+        self.crit_vec = 1e-8
+        self.crit_indp = 1e-12
+        self.lshift = 1e-4
+        self.nfac = 10
+        self.neig = 1
+        self.ndim = 0
+        self.diag = None        # device float64
+        self.HVec = None        # HVec(x_dev, out_dev)
+        self.nmvp = 0
+        self.niter = 0
+
+    def _ortho(self, ops, V, nb, R, rows, out_start):
+        """Orthonormalise R[rows] against V[:nb] (5 passes) into V[out_start:]."""
+        for _ in range(5):
+            for i in rows:
+                c = ops.dots(V, nb, R[i])
+                ops.lincomb(R[i], V, c, -1.0, 1.0)
+        nindp = 0
+        for pos, i in enumerate(rows):
+            rii = ops.norm(R[i])
+            if rii <= self.crit_indp:
+                continue
+            dst = V[out_start + nindp]
+            dst.copy_(R[i])
+            dst.mul_(1.0 / rii)
+            dst.mul_(1.0 / ops.norm(dst))
+            nindp += 1
+            for _ in range(5):
+                for j in rows[pos:]:
+                    c = ops.dots(V, nb, R[j])
+                    ops.lincomb(R[j], V, c, -1.0, 1.0)
+                    c2 = ops.dots(V[out_start:], nindp, R[j])
+                    ops.lincomb(R[j], V[out_start:], c2, -1.0, 1.0)
+        return nindp
+
+    def solve_iter(self, v0=None):
+        t = dv.torch()
+        n, neig = self.ndim, self.neig
+        if neig > n:
+            raise DavidsonError('neig > ndim')
+        ops = _VecOps(n)
+        maxv = self.nfac * neig + 2 * neig + 2
+        V = t.zeros((maxv, n), dtype=t.float64, device=dv.device())
+        W = t.zeros((maxv, n), dtype=t.float64, device=dv.device())
+        T = t.zeros((maxv, n), dtype=t.float64, device=dv.device())
+        R = t.zeros((neig, n), dtype=t.float64, device=dv.device())
+        self.nmvp = 0
+        if v0 is None:
+            d = dv.download(self.diag) + 1e-12 * np.arange(1, 1 + n) / float(n)
+            idx = np.argsort(d, kind='stable')[:neig]
+            for i, k in enumerate(idx):
+                V[i, int(k)] = 1.0
+        else:
+            V[:neig].copy_(v0 if not isinstance(v0, np.ndarray) else dv.upload(v0, 0).view(neig, n))
+        # genOrthoBas (pdvdson.py:335-345)
+        V[0].mul_(1.0 / ops.norm(V[0]))
+        if neig > 1:
+            R2 = V[1:neig].clone()
+            nindp = self._ortho(ops, V, 1, R2, list(range(neig - 1)), 1)
+            if nindp + 1 != neig:
+                raise DavidsonError('insufficient orthonormal basis')
+        for i in range(neig):
+            self.HVec(V[i], W[i])
+        self.nmvp += neig
+        nb = neig
+        eigs = np.zeros(neig) + 1e3
+        vr = None
+        for niter in range(1, self.maxcycle):
+            self.niter = niter
+            S = np.array([ops.dots(V, nb, V[i]) for i in range(nb)])
+            if np.linalg.norm(S - np.identity(nb)) > 1e-10:
+                raise DavidsonError('basis not orthonormal: %g' % np.linalg.norm(S - np.identity(nb)))
+            H = np.array([ops.dots(W, nb, V[i]) for i in range(nb)])
+            if np.linalg.norm(H - H.T) > 1e-10 * max(1.0, np.abs(H).max()):
+                raise DavidsonError('projected H not Hermitian: %g' % np.linalg.norm(H - H.T))
+            H = 0.5 * (H + H.T)
+            e, u = np.linalg.eigh(H)
+            vr = u[:, :neig].T.copy()
+            teig = e[:neig]
+            econv = [abs(teig[i] - eigs[i]) <= self.crit_e for i in range(neig)]
+            eigs = teig.copy()
+            rconv, rindx = [], []
+            for i in range(neig):
+                ops.lincomb(R[i], W, vr[i], 1.0, 0.0)
+                ops.lincomb(R[i], V, vr[i], -eigs[i], 1.0)
+                rn = ops.norm(R[i])
+                rconv.append(rn <= self.crit_vec)
+                if not rconv[i] and not econv[i]:
+                    rindx.append(i)
+            if not rindx or niter == self.maxcycle - 1:
+                break
+            if nb > self.nfac * neig:
+                pr = np.identity(nb)[:neig, :] - vr
+                nindp, vr2 = _host_ortho(vr, pr[rindx, :], self.crit_indp)
+                cm = np.vstack((vr, vr2)) if nindp else vr
+                for k in range(cm.shape[0]):
+                    ops.lincomb(T[k], V, cm[k], 1.0, 0.0)
+                V[:cm.shape[0]].copy_(T[:cm.shape[0]])
+                for k in range(cm.shape[0]):
+                    ops.lincomb(T[k], W, cm[k], 1.0, 0.0)
+                W[:cm.shape[0]].copy_(T[:cm.shape[0]])
+                nb = cm.shape[0]
+                vr = np.identity(nb)[:neig]
+            for i in range(neig):
+                if rconv[i]:
+                    continue
+                ops.precond(R[i], self.diag, eigs[i], self.lshift)
+            nindp = self._ortho(ops, V, nb, R, rindx, nb)
+            if nindp == 0:
+                raise DavidsonError('unable to generate new direction')
+            for k in range(nindp):
+                self.HVec(V[nb + k], W[nb + k])
+            self.nmvp += nindp
+            nb += nindp
+        vec = t.zeros((neig, n), dtype=t.float64, device=dv.device())
+        for i in range(neig):
+            ops.lincomb(vec[i], V, vr[i], 1.0, 0.0)
+        return eigs, vec, self.nmvp

@@ -1,0 +1,486 @@
+"""Block-sparse quantum-number tensor with device-resident values.
+
+Same block tables and value layout as the reference's qtensor
+(source/qtensor/qtensor.py:108-346): sectors per leg, C-ordered blocks, an
+allowed-block mask from the block rule (libzmpo ``zmpo_symAllowed``, the
+replacement of ctypes/qsym.c), values concatenated over allowed blocks in
+ascending block index.  Two differences, both internal to this package:
+
+* ``perm`` -- the storage order of the legs inside every block (identity =
+  the reference's row-major layout).  Environments and sigma intermediates
+  use non-identity orders so that the dominant GEMMs read K-contiguous
+  operands; ``toDenseTensor``/``fromDense`` honour it.
+* ``value`` -- a device tensor (torch, float64/float32) or a host numpy
+  array; ``tensordotCAL`` requires device values and runs every block pair
+  of the plan in one libzmpo grouped-GEMM launch.
+
+``tensordotSYM`` builds the same pair list as the reference (same output
+block order, same internal-tuple order, qtensor.py:563-637) with integer
+numpy, and lowers it to GEMM tasks (``lower_plan``).
+"""
+import numpy as np
+
+from . import _lib
+from . import device as dv
+
+
+def classification(qnums, ifclt=1):
+    """Sectors in first-occurrence order (qtensor_util.py:34-51)."""
+    rows = [tuple(float(x) for x in q) for q in qnums]
+    if not ifclt:
+        return (np.array([list(r) for r in rows], dtype=np.float64),
+                np.ones(len(rows), dtype=np.int64), [[i] for i in range(len(rows))])
+    order, groups = [], {}
+    for i, r in enumerate(rows):
+        if r not in groups:
+            groups[r] = []
+            order.append(r)
+        groups[r].append(i)
+    return (np.array([list(r) for r in order], dtype=np.float64),
+            np.array([len(groups[r]) for r in order], dtype=np.int64), [groups[r] for r in order])
+
+
+def fromQnums(qnums, ifcollect=None):
+    """qtensor_util.py:20-31."""
+    ifclt = [1] * len(qnums) if ifcollect is None else ifcollect
+    cls = [classification(q, c) for q, c in zip(qnums, ifclt)]
+    return len(qnums), [c[0] for c in cls], [c[1] for c in cls], [c[2] for c in cls]
+
+
+class QTensor:
+    def __init__(self, status=None, perm=None):
+        self.status = None if status is None else np.array(status, dtype=bool)
+        self.perm = None if perm is None else tuple(perm)
+        self.rank = None
+        self.qsyms = None
+        self.ndims = None
+        self.idlst = None
+        self.value = None
+        self.plan = None
+
+    # ---------------------------------------------------------- tables
+    def fromQsyms(self, rank, qsyms, ndims, ifallocate=True, idlst=None):
+        """qtensor.py:272-346 (mask from libzmpo)."""
+        self.rank = rank
+        self.qsyms = [np.array(q, dtype=np.float64).reshape(len(d), -1) for q, d in
+                      zip(qsyms, ndims)]
+        self.ndims = [np.array(d, dtype=np.int64) for d in ndims]
+        if idlst is not None:
+            self.idlst = [list(x) for x in idlst]
+        if self.perm is None:
+            self.perm = tuple(range(rank))
+        self.shape = np.array([int(d.sum()) for d in self.ndims], dtype=np.int64)
+        self.size = int(np.prod(self.shape))
+        self.nsyms = np.array([len(d) for d in self.ndims], dtype=np.int32)
+        self.nblks = int(np.prod(self.nsyms))
+        mask = _lib.sym_allowed(self.nsyms, self.status, self.qsyms)
+        self.idx_allowed = np.flatnonzero(mask)
+        self.nblks_allowed = len(self.idx_allowed)
+        ib = np.zeros(self.nblks, dtype=np.int32)
+        ib[self.idx_allowed] = self.idx_allowed + 1
+        self.iblks_allowed = ib.reshape(self.nsyms)
+        # per allowed block (position j): sector tuple, dims, size, offset
+        self.blk_isyms = np.array(np.unravel_index(self.idx_allowed, self.nsyms),
+                                  dtype=np.int64).T.reshape(-1, rank)
+        self.blk_dims = np.stack([self.ndims[i][self.blk_isyms[:, i]] for i in range(rank)],
+                                 axis=1) if self.nblks_allowed else np.zeros((0, rank), np.int64)
+        self.blk_size = np.prod(self.blk_dims, axis=1).astype(np.int64)
+        self.blk_off = np.concatenate([[0], np.cumsum(self.blk_size)[:-1]]).astype(np.int64) \
+            if self.nblks_allowed else np.zeros(0, np.int64)
+        self.size_allowed = int(self.blk_size.sum())
+        self.value = np.zeros(self.size_allowed) if ifallocate else None
+        return 0
+
+    def blk_strides(self):
+        """Element strides (nblks_allowed x rank) of each leg inside each block."""
+        st = np.zeros_like(self.blk_dims)
+        acc = np.ones(self.nblks_allowed, dtype=np.int64)
+        for leg in reversed(self.perm):
+            st[:, leg] = acc
+            acc = acc * self.blk_dims[:, leg]
+        return st
+
+    def same_structure(self, other):
+        return (self.rank == other.rank and np.array_equal(self.idx_allowed, other.idx_allowed)
+                and all(np.array_equal(a, b) for a, b in zip(self.ndims, other.ndims)))
+
+    def template(self, perm=None):
+        """Metadata copy (no values) with an optional new storage order."""
+        qt = QTensor(self.status, self.perm if perm is None else perm)
+        qt.fromQsyms(self.rank, self.qsyms, self.ndims, ifallocate=False, idlst=self.idlst)
+        return qt
+
+    # ---------------------------------------------------------- host values
+    def _host_value(self):
+        v = self.value
+        if v is None:
+            return None
+        if not isinstance(v, np.ndarray):
+            v = dv.download(v)
+        return v
+
+    def _ix(self, j, idlst):
+        isyms = self.blk_isyms[j]
+        return np.ix_(*[idlst[i][isyms[i]] for i in range(self.rank)])
+
+    def _dense_positions(self, idlst):
+        """(dense flat indices, value positions) of every stored entry, vectorised."""
+        shp = [int(x) for x in self.shape]
+        sec, loc = [], []
+        for i in range(self.rank):
+            s_i = np.zeros(shp[i], np.int64)
+            l_i = np.zeros(shp[i], np.int64)
+            for k, ids in enumerate(idlst[i]):
+                s_i[ids] = k
+                l_i[ids] = np.arange(len(ids))
+            sec.append(s_i)
+            loc.append(l_i)
+        bshape = [[1] * self.rank for _ in range(self.rank)]
+        for i in range(self.rank):
+            bshape[i][i] = shp[i]
+        nsy = [int(x) for x in self.nsyms]
+        flat = np.zeros(shp, np.int64)
+        mult = 1
+        for i in reversed(range(self.rank)):
+            flat = flat + sec[i].reshape(bshape[i]) * mult
+            mult *= nsy[i]
+        j = self.iblks_allowed.reshape(-1)[flat] - 1
+        mask = j >= 0
+        jm = np.searchsorted(self.idx_allowed, j[mask])
+        st = self.blk_strides()
+        pos = self.blk_off[jm].copy()
+        idx = np.nonzero(mask)
+        for i in range(self.rank):
+            pos += loc[i][idx[i]] * st[jm, i]
+        return np.flatnonzero(mask.reshape(-1)), pos
+
+    def fromDense(self, val, idlst):
+        """qtensor.py:364-379 (honours perm)."""
+        out = np.zeros(self.size_allowed)
+        if self.size <= 20_000_000:
+            d, pos = self._dense_positions(idlst)
+            out[pos] = np.asarray(val).reshape(-1)[d]
+        else:
+            for j in range(self.nblks_allowed):
+                blk = np.asarray(val[self._ix(j, idlst)]).transpose(self.perm)
+                out[self.blk_off[j]:self.blk_off[j] + self.blk_size[j]] = blk.reshape(-1)
+        self.value = out
+        return 0
+
+    def fromDenseTensor(self, val, qnums, ifcollect=None):
+        """qtensor.py:356-362."""
+        rank, qsyms, ndims, idlst = fromQnums(qnums, ifcollect)
+        self.fromQsyms(rank, qsyms, ndims, ifallocate=False)
+        self.fromDense(val, idlst)
+        self.idlst = [list(x) for x in idlst]
+        return 0
+
+    def toDenseTensor(self, idlst=None):
+        """qtensor.py:382-398 (honours perm)."""
+        idlst = self.idlst if idlst is None else idlst
+        v = self._host_value()
+        val = np.zeros(self.shape)
+        if self.size <= 20_000_000:
+            d, pos = self._dense_positions(idlst)
+            val.reshape(-1)[d] = v[pos]
+            return val
+        inv = np.argsort(self.perm)
+        for j in range(self.nblks_allowed):
+            dm = self.blk_dims[j]
+            blk = v[self.blk_off[j]:self.blk_off[j] + self.blk_size[j]].reshape(
+                dm[list(self.perm)]).transpose(inv)
+            val[self._ix(j, idlst)] = blk
+        return val
+
+    def toDenseDiag(self, idlst=None):
+        """Diagonal over the last two legs (qtensor.py:401-411)."""
+        full = self.toDenseTensor(idlst)
+        return np.einsum('...ii->...i', full)
+
+    def prjmap(self, idlst):
+        """Dense flat index of every stored entry, storage order (qtensor.py:442-453)."""
+        out = []
+        for j in range(self.nblks_allowed):
+            isyms = self.blk_isyms[j]
+            ind = [np.asarray(idlst[i][isyms[i]]) for i in range(self.rank)]
+            grids = np.meshgrid(*[ind[i] for i in self.perm], indexing='ij')
+            full = [None] * self.rank
+            for k, leg in enumerate(self.perm):
+                full[leg] = grids[k].reshape(-1)
+            out.append(np.ravel_multi_index(full, self.shape))
+        return np.concatenate(out) if out else np.zeros(0, dtype=np.int64)
+
+    # ---------------------------------------------------------- contraction
+    def tensordotSYM(self, qt1, qt2, axes, perm=None):
+        """Symbolic plan with the reference's pair order (qtensor.py:563-637).
+
+        ``perm`` sets the storage order of the output blocks.  Returns the
+        number of allowed output blocks.
+        """
+        i1, i2 = list(axes[0]), list(axes[1])
+        e1 = sorted(set(range(qt1.rank)) - set(i1))
+        e2 = sorted(set(range(qt2.rank)) - set(i2))
+        ne1, ne2 = len(e1), len(e2)
+        for a, b in zip(i1, i2):
+            if not np.array_equal(qt1.qsyms[a], qt2.qsyms[b]):
+                raise ValueError('tensordotSYM: contracted legs carry different sectors')
+        ints1 = [bool(qt1.status[i]) for i in i1]
+        ints2 = [bool(qt2.status[i]) for i in i2]
+        exts1 = [bool(qt1.status[i]) for i in e1]
+        exts2 = [bool(qt2.status[i]) for i in e2]
+        if i1:
+            if ints1[0] == ints2[0]:
+                ints2 = [not x for x in ints2]
+                exts2 = [not x for x in exts2]
+            if any(a == b for a, b in zip(ints1, ints2)):
+                raise ValueError('tensordotSYM: contracted legs must pair in/out')
+        self.status = np.array(exts1 + exts2, dtype=bool)
+        self.perm = tuple(range(ne1 + ne2)) if perm is None else tuple(perm)
+        self.fromQsyms(ne1 + ne2, [qt1.qsyms[i] for i in e1] + [qt2.qsyms[i] for i in e2],
+                       [qt1.ndims[i] for i in e1] + [qt2.ndims[i] for i in e2],
+                       ifallocate=False)
+        ns1 = qt1.nsyms
+        ns2 = qt2.nsyms
+        nint = int(np.prod([ns1[i] for i in i1])) if i1 else 1
+        ib1 = qt1.iblks_allowed.transpose(e1 + i1).reshape(-1, nint)
+        ib2 = qt2.iblks_allowed.transpose(e2 + i2).reshape(-1, nint)
+        iso = self.blk_isyms
+        x1 = (np.ravel_multi_index(iso[:, :ne1].T, [ns1[i] for i in e1]) if ne1
+              else np.zeros(len(iso), np.int64))
+        x2 = (np.ravel_multi_index(iso[:, ne1:].T, [ns2[i] for i in e2]) if ne2
+              else np.zeros(len(iso), np.int64))
+        sel = (ib1[x1] > 0) & (ib2[x2] > 0)
+        o, k = np.nonzero(sel)
+        a_flat = ib1[x1[o], k].astype(np.int64) - 1
+        b_flat = ib2[x2[o], k].astype(np.int64) - 1
+        self.plan = Plan(qt1, qt2, self, (i1, i2), e1, e2, o.astype(np.int64),
+                         np.searchsorted(qt1.idx_allowed, a_flat),
+                         np.searchsorted(qt2.idx_allowed, b_flat))
+        return self.nblks_allowed
+
+    def tensordotCAL(self, qt1, qt2, ifc1=False, ifc2=False, thresh=1e-20, tsize=10000,
+                     dtype=_lib.F64):
+        """Run the plan on the device in one grouped-GEMM launch (qtensor.py:642-659).
+
+        Real arithmetic: ifc1/ifc2 (complex conjugation) are no-ops.  The
+        reference's skip of all-zero blocks (< thresh, > tsize entries) is
+        not applied; such blocks contribute nothing.
+        """
+        if self.plan is None:
+            raise ValueError('tensordotCAL before tensordotSYM')
+        prog = self.plan.lowered()
+        out = dv.zeros(self.size_allowed, dtype)
+        a = _as_device(qt1.value, dtype)
+        b = _as_device(qt2.value, dtype)
+        run_program(prog, a, b, out, dtype=dtype, beta=0)
+        self.value = out
+        return 0
+
+
+def _as_device(v, dtype):
+    if isinstance(v, np.ndarray):
+        return dv.upload(v, dtype)
+    want = dv.tdtype(dtype)
+    return v if v.dtype == want else v.to(want)
+
+
+def _mergeable(qt, outer, inner, always_unit):
+    """Legs outer, inner adjacent in qt's storage order (ignoring always-unit legs)."""
+    p = list(qt.perm)
+    a, b = p.index(outer), p.index(inner)
+    if a >= b:
+        return False
+    return all(always_unit[x] for x in p[a + 1:b])
+
+
+class Plan:
+    """Pair list of one contraction: rows (out block j, qt1 block ja, qt2 block jb)."""
+
+    def __init__(self, qt1, qt2, out, axes, e1, e2, jo, ja, jb):
+        self.qt1, self.qt2, self.out = qt1, qt2, out
+        self.axes, self.e1, self.e2 = axes, e1, e2
+        self.jo, self.ja, self.jb = jo, ja, jb
+        self._lowered = None
+
+    def flops(self):
+        """Algorithmic FLOPs: sum over pairs of 2 * free1 * contracted * free2."""
+        lw = self.lowered()
+        p = lw['pairs']
+        t = lw['tasks']
+        rows_task = np.repeat(np.arange(len(t)), (t['p_end'] - t['p_begin']).astype(np.int64))
+        return int(np.sum(2 * t['M'][rows_task].astype(np.int64) * t['N'][rows_task]
+                          * p['K'].astype(np.int64)))
+
+    def lowered(self):
+        if self._lowered is None:
+            self._lowered = lower_plan(self)
+        return self._lowered
+
+
+def _unit_legs(qt):
+    return [bool(np.all(qt.blk_dims[:, i] <= 1)) if qt.nblks_allowed else True
+            for i in range(qt.rank)]
+
+
+def _groups(legs_x, legs_c, qx, qc, ux, uc):
+    """Split a free-leg list into runs that are storage-adjacent in both X and C."""
+    groups = [[0]] if legs_x else []
+    for j in range(1, len(legs_x)):
+        if (_mergeable(qx, legs_x[j - 1], legs_x[j], ux)
+                and _mergeable(qc, legs_c[j - 1], legs_c[j], uc)):
+            groups[-1].append(j)
+        else:
+            groups.append([j])
+    if len(groups) > 2:
+        raise NotImplementedError('more than two independent free legs in one block GEMM')
+    return groups
+
+
+def _apply(groups, dims, strides):
+    """(lo_dim, hi_stride, lo_stride, total) of a <=2-level index over merged legs."""
+    nrow = dims.shape[0]
+    one = np.ones(nrow, np.int64)
+    zero = np.zeros(nrow, np.int64)
+    if not groups:
+        return one, zero, zero, one
+    gd = [np.prod(dims[:, g], axis=1) for g in groups]
+    gs = [strides[:, g[-1]] for g in groups]
+    if len(groups) == 1:
+        return gd[0], zero, gs[0], gd[0]
+    return gd[1], gs[0], gs[1], gd[0] * gd[1]
+
+
+def lower_plan(plan):
+    """Lower a pair list to libzmpo tasks/pairs (offsets relative to each value array)."""
+    qa, qb, qc = plan.qt1, plan.qt2, plan.out
+    i1, i2 = plan.axes
+    e1, e2 = plan.e1, plan.e2
+    ua, ub, uc = _unit_legs(qa), _unit_legs(qb), _unit_legs(qc)
+    sa, sb, sc = qa.blk_strides(), qb.blk_strides(), qc.blk_strides()
+    jo, ja, jb = plan.jo, plan.ja, plan.jb
+    da, db = qa.blk_dims[ja], qb.blk_dims[jb]
+    # free legs of A <-> out legs 0..ne1-1 ; free legs of B <-> out legs ne1..
+    mA = [l for l in e1 if not (ua[l] and uc[e1.index(l)])]
+    mC = [e1.index(l) for l in mA]
+    nB = [l for l in e2 if not (ub[l] and uc[len(e1) + e2.index(l)])]
+    nC = [len(e1) + e2.index(l) for l in nB]
+    gm = _groups(mA, mC, qa, qc, ua, uc)
+    gn = _groups(nB, nC, qb, qc, ub, uc)
+    # contracted legs (pairs), active ones only, merged into one K index
+    kp = [(a, b) for a, b in zip(i1, i2) if not (ua[a] and ub[b])]
+    if len(kp) > 1:
+        order = None
+        for cand in (kp, kp[::-1]):
+            if all(_mergeable(qa, cand[j - 1][0], cand[j][0], ua)
+                   and _mergeable(qb, cand[j - 1][1], cand[j][1], ub)
+                   for j in range(1, len(cand))):
+                order = cand
+                break
+        if order is None:
+            raise NotImplementedError('contracted legs not mergeable in both operands')
+        kp = order
+    nrow = len(jo)
+    if kp:
+        K = np.prod(da[:, [a for a, _ in kp]], axis=1)
+        a_sk = sa[ja][:, kp[-1][0]]
+        b_sk = sb[jb][:, kp[-1][1]]
+    else:
+        K = np.ones(nrow, np.int64)
+        a_sk = np.zeros(nrow, np.int64)
+        b_sk = np.zeros(nrow, np.int64)
+    a_mlo, a_hi, a_lo, _ = _apply(gm, da[:, mA], sa[ja][:, mA])
+    b_nlo, b_hi, b_lo, _ = _apply(gn, db[:, nB], sb[jb][:, nB])
+    # task (output block) geometry from the output tensor, same leg groups
+    nout = qc.nblks_allowed
+    dc = qc.blk_dims
+    scs = qc.blk_strides()
+    c_mlo, c_hi, c_lo, Mt = _apply(gm, dc[:, mC], scs[:, mC])
+    n_lo, cn_hi, cn_lo, Nt = _apply(gn, dc[:, nC], scs[:, nC])
+    if len(gn) == 2 and np.any((Nt > n_lo) & (cn_hi != cn_lo * n_lo)):
+        raise NotImplementedError('output N index is not single-strided')
+    pairs = np.zeros(nrow, dtype=_lib.PAIR_DTYPE)
+    pairs['a_off'] = qa.blk_off[ja]
+    pairs['b_off'] = qb.blk_off[jb]
+    pairs['scale'] = 1.0
+    pairs['K'] = K
+    pairs['a_mlo'] = np.maximum(a_mlo, 1)
+    pairs['a_smhi'] = a_hi
+    pairs['a_smlo'] = a_lo
+    pairs['a_sk'] = a_sk
+    pairs['b_nlo'] = np.maximum(b_nlo, 1)
+    pairs['b_snhi'] = b_hi
+    pairs['b_snlo'] = b_lo
+    pairs['b_sk'] = b_sk
+    counts = np.bincount(jo, minlength=nout)
+    pbeg = np.concatenate([[0], np.cumsum(counts)[:-1]]) if nout else np.zeros(0, np.int64)
+    tasks = np.zeros(nout, dtype=_lib.TASK_DTYPE)
+    tasks['c_off'] = qc.blk_off
+    tasks['M'] = Mt
+    tasks['N'] = Nt
+    tasks['c_mlo'] = np.maximum(c_mlo, 1)
+    tasks['c_smhi'] = c_hi
+    tasks['c_smlo'] = c_lo
+    tasks['c_sn'] = cn_lo if len(gn) else 0
+    tasks['p_begin'] = pbeg
+    tasks['p_end'] = pbeg + counts
+    finish_tasks(tasks)
+    return {'tasks': tasks, 'pairs': pairs}
+
+
+def finish_tasks(tasks):
+    """Fill tiles_n and tile_base (exclusive prefix sum of 64x64 tiles)."""
+    T = _lib.TILE
+    tm = (tasks['M'].astype(np.int64) + T - 1) // T
+    tn = (tasks['N'].astype(np.int64) + T - 1) // T
+    tasks['tiles_n'] = np.maximum(tn, 1)
+    nt = np.maximum(tm, 1) * np.maximum(tn, 1)
+    tasks['tile_base'] = np.concatenate([[0], np.cumsum(nt)[:-1]]) if len(tasks) else 0
+    return int(nt.sum()) if len(tasks) else 0
+
+
+class Program:
+    """Task/pair arrays of one grouped-GEMM launch (uploaded on first use)."""
+
+    def __init__(self, tasks, pairs):
+        self.ntasks = len(tasks)
+        self.npairs = len(pairs)
+        self.ntiles = finish_tasks(tasks) if self.ntasks else 0
+        self.tasks_host, self.pairs_host = tasks, pairs
+        self._dev = None
+        rows = np.repeat(np.arange(self.ntasks), (tasks['p_end'] - tasks['p_begin']).astype(np.int64))
+        self.flops = int(np.sum(2 * tasks['M'][rows].astype(np.int64) * tasks['N'][rows]
+                                * pairs['K'].astype(np.int64))) if self.npairs else 0
+
+    def device(self):
+        if self._dev is None:
+            self._dev = (dv.upload_bytes(self.tasks_host), dv.upload_bytes(self.pairs_host))
+        return self._dev
+
+
+def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None):
+    """Launch one grouped block GEMM: C (=|+=) sum over pairs of A_pair B_pair."""
+    if isinstance(prog, dict):
+        prog = Program(prog['tasks'], prog['pairs'])
+    if prog.ntasks == 0:
+        return
+    tasks, pairs = prog.device()
+    L = _lib.lib()
+    _lib.check(L.zmpo_blockgemm(dtype, dv.ptr(tasks), prog.ntasks, dv.ptr(pairs),
+                                prog.ntiles, dv.ptr(A), dv.ptr(B), dv.ptr(C), beta,
+                                dv.stream() if stream is None else stream))
+
+
+def gather(dst, src, idx, dtype_dst=_lib.F64, dtype_src=_lib.F64):
+    """dst[i] = src[idx[i]] on the device (idx < 0 gives 0)."""
+    n = idx.numel()
+    _lib.check(_lib.lib().zmpo_gather(dtype_dst, dv.ptr(dst), dtype_src, dv.ptr(src), dv.ptr(idx),
+                                      n, dv.stream()))
+    return dst
+
+
+def tensordot(qt1, qt2, axes, perm=None, dtype=_lib.F64):
+    """One-shot contraction (qtensor.py:665-827 general case) on the device."""
+    out = QTensor()
+    out.tensordotSYM(qt1, qt2, axes, perm=perm)
+    out.tensordotCAL(qt1, qt2, dtype=dtype)
+    return out
