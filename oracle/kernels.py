@@ -63,7 +63,16 @@ def HVecQt(vec0, st, lenv, renv, isite, stats=None):
     qvec.value = np.array(vec0, dtype=np.float64).copy()
     hvec = np.zeros(len(vec0))
     jsite = isite + 1
-    top1, top2, top3, top4 = {}, {}, {}, {}
+    # The reference rebuilds the symbolic plans on every call (kernelQt.py:167-174);
+    # they depend only on the block structure, so they are kept for the whole
+    # Davidson solve of one site (identical results, less Python time).
+    cache = getattr(st, '_hvec_plans', None)
+    if cache is not None and cache[0] is qvec and cache[1] == isite and cache[2] is lenv \
+            and cache[3] is renv:
+        top1, top2, top3, top4 = cache[4]
+    else:
+        top1, top2, top3, top4 = {}, {}, {}, {}
+        st._hvec_plans = (qvec, isite, lenv, renv, (top1, top2, top3, top4))
     for iop in range(st.nops):
         p = st.opers[iop]
         c0 = case(p, isite, 2)

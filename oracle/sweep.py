@@ -15,7 +15,10 @@ rule depends on Python-2 dict order and an unstable argsort and is not
 reproducible, mpo_dmrg_qparser.py:220-233,259): weights sorted descending;
 runs of weights equal to within 1e-10 relative (or 1e-14 absolute) form a
 tie cluster, ordered inside by (N, Sz) ascending then by position; keep the
-first min(Dcut, n) states; re-sort the kept states by (N, Sz), stable.
+first min(Dcut, n) states; re-sort the kept states by (N, Sz), stable.  The
+reference keeps zero-weight states (thresh = -1e-14), whose eigenvectors are
+an arbitrary basis of the RDM null space; here that basis is fixed
+(canonical_null) so that two implementations agree beyond rounding.
 """
 import numpy as np
 
@@ -48,6 +51,34 @@ def truncation_order(sigs, qsyms):
     return np.array(out, dtype=np.int64)
 
 
+def canonical_null(u, s2, tol):
+    """Replace eigenvectors with weight <= tol by a fixed basis of their span:
+    unit vectors e_0, e_1, ... projected out of the kept vectors and of each
+    other, accepted when the projected norm exceeds 0.5 (then 1e-6)."""
+    keep = s2 > tol
+    nnull = int(np.sum(~keep))
+    if nnull == 0:
+        return u
+    basis = [u[:, keep]]
+    out = []
+    n = u.shape[0]
+    for cut in (0.5, 1e-6):
+        for j in range(n):
+            if len(out) == nnull:
+                break
+            v = np.zeros(n)
+            v[j] = 1.0
+            for _ in range(2):
+                for b in basis + ([np.array(out).T] if out else []):
+                    v = v - b @ (b.T @ v)
+            nv = np.linalg.norm(v)
+            if nv > cut:
+                out.append(v / nv)
+    u2 = u.copy()
+    u2[:, ~keep] = np.array(out).T
+    return u2
+
+
 def rdm_blkdiag(rho, classes, thresh, Dcut):
     """Per-sector eigh of the RDM and truncation (mpo_dmrg_qparser.py:101-180)."""
     groups = {}
@@ -64,6 +95,7 @@ def rdm_blkdiag(rho, classes, thresh, Dcut):
         rows = groups[k]
         s2, u = np.linalg.eigh(rho[np.ix_(rows, rows)])
         s2 = np.abs(s2)
+        u = canonical_null(u, s2, 1e-14)
         uc = np.zeros((nrows, len(rows)))
         uc[rows, :] = u
         rot.append(uc)

@@ -8,7 +8,12 @@ import contextlib
 import numpy as np
 
 
+def _np(x):
+    return x.numpy() if hasattr(x, 'numpy') else x
+
+
 def emulate(prog, A, B, C, dtype=0, beta=0, stream=None):
+    A, B, C = _np(A), _np(B), _np(C)
     if isinstance(prog, dict):
         from zmpo_dmrg_amd.qtensor import Program
         prog = Program(prog['tasks'], prog['pairs'])
@@ -32,15 +37,43 @@ def emulate(prog, A, B, C, dtype=0, beta=0, stream=None):
 
 
 def _gather(dst, src, idx, *a):
-    dst[:] = np.where(idx >= 0, src[np.maximum(idx, 0)], 0.0)
+    d, s_, i = _np(dst), _np(src), _np(idx)
+    d[:] = np.where(i >= 0, s_[np.maximum(i, 0)], 0.0)
     return dst
+
+
+class _CpuVecOps:
+    """numpy stand-in for pdvdson._VecOps (same fixed-order semantics)."""
+
+    def __init__(self, n):
+        self.n = n
+
+    def dots(self, X, nx, y):
+        return np.array([float(_np(X[j]) @ _np(y)) for j in range(nx)])
+
+    def lincomb(self, out, X, coef, alpha=1.0, beta=0.0):
+        o = _np(out)
+        acc = np.zeros(self.n)
+        for j, c in enumerate(coef):
+            acc += c * _np(X[j])
+        o[:] = alpha * acc if beta == 0.0 else beta * o + alpha * acc
+        return out
+
+    def norm(self, x):
+        return float(np.linalg.norm(_np(x)))
+
+    def precond(self, r, diag, e, shift):
+        rr = _np(r)
+        rr[:] = rr / (np.abs(_np(diag) - e) + shift)
 
 
 @contextlib.contextmanager
 def host_model():
     """Run the zmpo_dmrg_amd engine on numpy buffers with the CPU kernel model."""
+    import torch
     import zmpo_dmrg_amd.device as dv
     import zmpo_dmrg_amd.kernelQt as kq
+    import zmpo_dmrg_amd.pdvdson as pd
     import zmpo_dmrg_amd.qtensor as qq
     saved = {}
 
@@ -48,10 +81,17 @@ def host_model():
         saved[(mod, name)] = getattr(mod, name)
         setattr(mod, name, fn)
 
-    patch(dv, 'empty', lambda n, code=0: np.zeros(int(max(n, 1))))
-    patch(dv, 'zeros', lambda n, code=0: np.zeros(int(max(n, 1))))
-    patch(dv, 'upload', lambda a, code=None: np.array(a, dtype=np.float64 if a.dtype.kind == 'f' else a.dtype))
-    patch(dv, 'download', lambda x: np.array(x))
+    def upload(a, code=None):
+        a = np.asarray(a)
+        return torch.from_numpy(np.array(a, dtype=np.float64 if a.dtype.kind == 'f' else a.dtype))
+
+    patch(dv, 'empty', lambda n, code=0: torch.zeros(int(max(n, 1)), dtype=torch.float64))
+    patch(dv, 'zeros', lambda n, code=0: torch.zeros(int(max(n, 1)), dtype=torch.float64))
+    patch(dv, 'upload', upload)
+    patch(dv, 'download', lambda x: np.array(_np(x)))
+    patch(dv, 'device', lambda: torch.device('cpu'))
+    patch(dv, 'require', lambda: None)
+    patch(pd, '_VecOps', _CpuVecOps)
     patch(qq, 'run_program', emulate)
     patch(kq, 'run_program', emulate)
     patch(qq, 'gather', _gather)

@@ -22,10 +22,11 @@ def to_product(o):
 class Case:
     """Oracle state after ``sweeps`` sweeps at (K, D); product MPO/MPS/envs mirror it."""
 
-    def __init__(self, K=4, D=16, seed=3, sweeps=1, upload=False):
+    def __init__(self, K=4, D=16, seed=3, sweeps=1, upload=False, nelec=None):
         self.K = K
+        self.nelec = (K if K % 2 == 0 else K - 1) if nelec is None else nelec
         self.i1, self.i2 = gen_integrals(K, seed=seed)
-        self.st = osw.State(self.i1, self.i2, K, 0.0, Dmax=D, tol=1e-10)
+        self.st = osw.State(self.i1, self.i2, self.nelec, 0.0, Dmax=D, tol=1e-10)
         for _ in range(sweeps):
             osw.sweep(self.st, 'R')
             osw.sweep(self.st, 'L')
@@ -45,7 +46,7 @@ class Case:
         return L, R
 
     def templates(self, isite):
-        qkey = np.array([float(self.K), 0.0])
+        qkey = np.array([float(self.nelec), 0.0])
         qp = pmpo.qphys()
         qnums = [self.st.qnuml[isite], qp, qp, qkey - self.st.qnumr[isite + 2]]
         oq = OQ([False, False, False, True])

@@ -87,10 +87,10 @@ def test_device_dmrg_matches_oracle_sweep_by_sweep(gpu):
     """Truncated run (D below the full space): energies per half sweep and the
     bond QN structure after truncation identical to the oracle."""
     from oracle import sweep as osw
-    K, D = 5, 12
+    K, D, N = 5, 12, 4
     i1, i2 = gen_integrals(K, seed=5)
-    st = osw.State(i1, i2, K, 0.0, Dmax=D, tol=1e-10)
-    d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=D, tol=1e-10)
+    st = osw.State(i1, i2, N, 0.0, Dmax=D, tol=1e-10)
+    d = pdmrg.MPODMRG(i1, i2, N, 0.0, Dmax=D, tol=1e-10)
     for _ in range(2):
         for status in ('R', 'L'):
             ro = osw.sweep(st, status)
@@ -101,3 +101,27 @@ def test_device_dmrg_matches_oracle_sweep_by_sweep(gpu):
             for i in range(K + 1):
                 assert np.array_equal(np.asarray(st.qnuml[i]), np.asarray(d.qnuml[i]))
                 assert np.array_equal(np.asarray(st.qnumr[i]), np.asarray(d.qnumr[i]))
+
+
+@pytest.mark.parametrize('name', ['k6d20', 'cfg1'])
+def test_device_dmrg_matches_oracle_fixture(gpu, name):
+    """Full device sweeps vs the stored oracle run: every step's eigenvalue to
+    1e-8 Ha and every bond's (N, Sz) list after truncation identical."""
+    path = os.path.join(GOLDEN, 'dmrg_%s.npz' % name)
+    if not os.path.exists(path):
+        pytest.skip('fixture %s not generated' % name)
+    z = np.load(path)
+    K, D, seed = int(z['K']), int(z['D']), int(z['seed'])
+    i1, i2 = gen_integrals(K, seed=seed)
+    d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=D, tol=1e-8)
+    eig = z['eig']
+    k = 0
+    for _ in range(eig.shape[0] // 2):
+        for status in ('R', 'L'):
+            r = pdmrg.sweep(d, status)
+            ep = np.array([x[0] for x in r[3]])
+            assert np.abs(ep - eig[k]).max() < 1e-8, (status, k, np.abs(ep - eig[k]).max())
+            k += 1
+    for i in range(K + 1):
+        assert np.array_equal(np.asarray(d.qnuml[i]), z['qnuml%d' % i]), i
+        assert np.array_equal(np.asarray(d.qnumr[i]), z['qnumr%d' % i]), i

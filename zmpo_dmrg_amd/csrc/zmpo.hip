@@ -41,12 +41,16 @@ typedef float flt4 __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct Acc;
+// Result element r of lane l lands at row(l/16, r), column l%16 of the 16x16
+// tile: FP64 interleaves rows across lane groups, FP32 keeps four consecutive
+// rows per lane group.
 template <>
 struct Acc<double> {
   typedef dbl4 type;
   __device__ static inline dbl4 mfma(double a, double b, dbl4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lk, int r) { return lk + 4 * r; }
 };
 template <>
 struct Acc<float> {
@@ -54,6 +58,7 @@ struct Acc<float> {
   __device__ static inline flt4 mfma(float a, float b, flt4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lk, int r) { return 4 * lk + r; }
 };
 
 constexpr int TM = 64;
@@ -74,7 +79,7 @@ __device__ inline int find_task(const zmpo_task* __restrict__ tasks, int ntasks,
 // C[task] (=|+=) sum_pairs scale * A_pair(M x K) * B_pair(K x N).
 // 4 waves in a 2x2 grid, each wave owns a 32x32 quadrant = 2x2 MFMA 16x16
 // accumulators.  MFMA 16x16x4 operand layout: lane l holds A[l%16][l/16] and
-// B[l/16][l%16]; result element r of lane l is D[4*(l/16)+r][l%16].
+// B[l/16][l%16]; result element r of lane l is D[Acc<T>::row(l/16, r)][l%16].
 template <typename T>
 __global__ __launch_bounds__(NTHREADS) void blockgemm_kernel(
     const zmpo_task* __restrict__ tasks, int ntasks, const zmpo_pair* __restrict__ pairs,
@@ -180,7 +185,7 @@ __global__ __launch_bounds__(NTHREADS) void blockgemm_kernel(
     // Epilogue: one owner per output element.
     for (int rb = 0; rb < 2; ++rb) {
       for (int r = 0; r < 4; ++r) {
-        const int mi = m0 + wm * 32 + rb * 16 + 4 * lk + r;
+        const int mi = m0 + wm * 32 + rb * 16 + Acc<T>::row(lk, r);
         if (mi >= tk.M) continue;
         const int64_t moff = tk.c_off + (int64_t)(mi / tk.c_mlo) * tk.c_smhi +
                              (int64_t)(mi % tk.c_mlo) * tk.c_smlo;

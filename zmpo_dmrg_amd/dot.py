@@ -11,8 +11,11 @@ sectors (per-sector eigh, as mpo_dmrg_qparser.rdm_blkdiag); its truncation
 rule is deterministic: weights sorted descending; runs of weights equal to
 1e-10 relative (1e-14 absolute) form tie clusters ordered by (N, Sz) then
 position; keep the first min(Dcut, n); re-sort the kept states by (N, Sz).
-The reference's own rule depends on Python-2 dict order and an unstable
-argsort (mpo_dmrg_qparser.py:220-233, 259) and cannot be reproduced.
+Zero-weight states (kept by the reference, thresh = -1e-14) get a fixed
+basis of the RDM null space (canonical_null) instead of LAPACK's arbitrary
+one.  The reference's own rule depends on Python-2 dict order and an
+unstable argsort (mpo_dmrg_qparser.py:220-233, 259) and cannot be
+reproduced.
 """
 import numpy as np
 
@@ -119,6 +122,37 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
     return nmvps, np.array(eigs), sigs, dwts, qred, site, srotR
 
 
+def canonical_null(u, s2, tol):
+    """Fixed basis for the RDM null space of one sector: eigenvectors with
+    weight <= tol are replaced by unit vectors e_0, e_1, ... orthogonalised
+    against the kept vectors and each other (twice), accepted when their
+    projected norm exceeds 0.5 (second pass 1e-6)."""
+    keep = s2 > tol
+    nnull = int(np.sum(~keep))
+    if nnull == 0:
+        return u
+    kept = u[:, keep]
+    out = []
+    n = u.shape[0]
+    for cut in (0.5, 1e-6):
+        for j in range(n):
+            if len(out) == nnull:
+                break
+            v = np.zeros(n)
+            v[j] = 1.0
+            for _ in range(2):
+                v = v - kept @ (kept.T @ v)
+                if out:
+                    ob = np.array(out).T
+                    v = v - ob @ (ob.T @ v)
+            nv = np.linalg.norm(v)
+            if nv > cut:
+                out.append(v / nv)
+    u2 = u.copy()
+    u2[:, ~keep] = np.array(out).T
+    return u2
+
+
 def rdm_blkdiag(mats, classes, thresh, Dcut, side):
     """Per-sector RDM eigh and truncation (mpo_dmrg_qparser.py:101-180).
 
@@ -155,6 +189,7 @@ def rdm_blkdiag(mats, classes, thresh, Dcut, side):
             rho = rho / tr
         s2, u = np.linalg.eigh(rho)
         s2 = np.abs(s2)
+        u = canonical_null(u, s2, 1e-14)
         for c in range(len(rows)):
             rot_cols.append((rows, u[:, c]))
         sigs.append(s2)
