@@ -19,6 +19,9 @@
  *                        renormQt_H00 (kernelQt.py:674-683, 752-758) and
  *                        genHopsQt (source/mpo_dmrg_initQt.py:146-152), and
  *                        the final contraction of HDiagQt (kernelQt.py:116-119).
+ *   zmpo_wavegemm        the same numeric loop, scheduled one wave per small
+ *                        output tile (the block shapes of block-sparse
+ *                        DMRG are mostly 5-150 wide).
  *   zmpo_dots, zmpo_lincomb, zmpo_precond
  *                        the vector algebra of the Davidson solver
  *                        (source/tools/pdvdson.py:131-283, 350-390), which the
@@ -71,6 +74,19 @@ typedef struct zmpo_pair {
   int32_t pad;
 } zmpo_pair;
 
+/* One wave's work item for zmpo_wavegemm: the (16*rb) x 16 tile at (m0, n0)
+ * of task `task`, over pairs [p_begin, p_end).  slot < 0 writes the tile to C;
+ * slot >= 0 writes it to work[slot * rb * 256 ...] for zmpo_rgroup reduction. */
+typedef struct zmpo_unit {
+  int32_t task, m0, n0, p_begin, p_end, slot, pad0, pad1;
+} zmpo_unit;
+
+/* Sum of work slots [slot_begin, slot_begin + nslots) into the tile (m0, n0)
+ * of task `task`, in slot order. */
+typedef struct zmpo_rgroup {
+  int32_t task, m0, n0, slot_begin, nslots, pad;
+} zmpo_rgroup;
+
 const char* zmpo_last_error(void);
 int zmpo_abi_version(void);
 int zmpo_struct_sizes(int* task_size, int* pair_size);
@@ -93,6 +109,14 @@ int zmpo_symAllowedInt(int rank, int nqnum, const int* ndims, const int* status,
  * dtype ZMPO_F64 (FP64 MFMA) or ZMPO_F32 (FP32 MFMA); A, B, C share it. */
 int zmpo_blockgemm(int dtype, const zmpo_task* tasks, int64_t ntasks, const zmpo_pair* pairs,
                    int64_t ntiles, const void* A, const void* B, void* C, int beta, void* stream);
+
+/* Small-block grouped GEMM: one wave per zmpo_unit (rb in {1, 2, 4} row
+ * blocks of 16), then the deterministic reduction of split tiles.  Same
+ * task/pair semantics as zmpo_blockgemm; work holds the split tiles (T). */
+int zmpo_wavegemm(int dtype, int rb, const zmpo_task* tasks, const zmpo_unit* units,
+                  int64_t nunits, const zmpo_rgroup* groups, int64_t ngroups,
+                  const zmpo_pair* pairs, const void* A, const void* B, void* C, void* work,
+                  int beta, void* stream);
 
 /* out[j] = sum_i X[j*ldx + i] * y[i] for j < nx (device out, float64;
  * work holds zmpo_dots_workspace(nx) doubles).  Fixed reduction order. */

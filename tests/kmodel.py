@@ -36,6 +36,55 @@ def emulate(prog, A, B, C, dtype=0, beta=0, stream=None):
         C[cidx] = acc if beta == 0 else C[cidx] + acc
 
 
+def emulate_wave(prog, A, B, C, dtype=0, beta=0, stream=None, kernel='wave'):
+    """Model of zmpo_wavegemm: per-unit tiles over pair chunks, split tiles
+    summed per group in slot order (the kernel's own schedule)."""
+    from zmpo_dmrg_amd.qtensor import Program, make_schedule
+    if isinstance(prog, dict):
+        prog = Program(prog['tasks'], prog['pairs'])
+    if kernel == 'tile':
+        return emulate(prog, A, B, C, dtype, beta)
+    A, B, C = _np(A), _np(B), _np(C)
+    rb = prog.rb
+    units, groups, nslots = make_schedule(prog.tasks_host, prog.pairs_host, rb)
+    t, p = prog.tasks_host, prog.pairs_host
+    work = np.zeros((max(nslots, 1), 16 * rb, 16))
+
+    def cidx(tk, m, n):
+        return (tk['c_off'] + (m // tk['c_mlo']) * tk['c_smhi']
+                + (m % tk['c_mlo']) * tk['c_smlo'])[:, None] + n[None, :] * tk['c_sn']
+
+    for u in units:
+        tk = t[u['task']]
+        m = u['m0'] + np.arange(16 * rb)
+        n = u['n0'] + np.arange(16)
+        mv, nv = m < tk['M'], n < tk['N']
+        acc = np.zeros((16 * rb, 16))
+        for pi in range(int(u['p_begin']), int(u['p_end'])):
+            q = p[pi]
+            k = np.arange(int(q['K']))
+            ai = q['a_off'] + (m // q['a_mlo']) * q['a_smhi'] + (m % q['a_mlo']) * q['a_smlo']
+            bi = q['b_off'] + (n // q['b_nlo']) * q['b_snhi'] + (n % q['b_nlo']) * q['b_snlo']
+            Am = A[np.where(mv, ai, 0)[:, None] + k[None, :] * q['a_sk']] * mv[:, None]
+            Bm = B[k[:, None] * q['b_sk'] + np.where(nv, bi, 0)[None, :]] * nv[None, :]
+            acc += q['scale'] * (Am @ Bm)
+        if u['slot'] >= 0:
+            work[u['slot']] = acc
+        else:
+            ci = cidx(tk, m[mv], n[nv])
+            v = acc[np.ix_(mv, nv)]
+            C[ci] = v if beta == 0 else C[ci] + v
+    for g in groups:
+        tk = t[g['task']]
+        s_ = work[g['slot_begin']:g['slot_begin'] + g['nslots']].sum(0)
+        m = g['m0'] + np.arange(16 * rb)
+        n = g['n0'] + np.arange(16)
+        mv, nv = m < tk['M'], n < tk['N']
+        ci = cidx(tk, m[mv], n[nv])
+        v = s_[np.ix_(mv, nv)]
+        C[ci] = v if beta == 0 else C[ci] + v
+
+
 def _gather(dst, src, idx, *a):
     d, s_, i = _np(dst), _np(src), _np(idx)
     d[:] = np.where(i >= 0, s_[np.maximum(i, 0)], 0.0)
@@ -92,8 +141,8 @@ def host_model():
     patch(dv, 'device', lambda: torch.device('cpu'))
     patch(dv, 'require', lambda: None)
     patch(pd, '_VecOps', _CpuVecOps)
-    patch(qq, 'run_program', emulate)
-    patch(kq, 'run_program', emulate)
+    patch(qq, 'run_program', emulate_wave)
+    patch(kq, 'run_program', emulate_wave)
     patch(qq, 'gather', _gather)
     patch(kq, 'gather', _gather)
     try:

@@ -21,16 +21,18 @@ def _dense_program(M, N, K, trans_a=False, trans_b=False):
     return pq.Program(t, p)
 
 
-@pytest.mark.parametrize('shape', [(64, 64, 16), (37, 90, 33), (130, 7, 200), (1, 1, 1)])
+@pytest.mark.parametrize('shape', [(64, 64, 16), (37, 90, 33), (130, 7, 200), (1, 1, 1),
+                                   (5, 9, 5000)])
 @pytest.mark.parametrize('ta,tb', [(False, False), (True, False), (False, True)])
-def test_blockgemm_dense_f64(gpu, shape, ta, tb):
+@pytest.mark.parametrize('kernel', ['wave', 'tile'])
+def test_blockgemm_dense_f64(gpu, shape, ta, tb, kernel):
     M, N, K = shape
     rng = np.random.default_rng(M * N + K)
     A = rng.standard_normal(M * K)
     B = rng.standard_normal(K * N)
     prog = _dense_program(M, N, K, ta, tb)
     C = dv.zeros(M * N, 0)
-    pq.run_program(prog, dv.upload(A, 0), dv.upload(B, 0), C, 0)
+    pq.run_program(prog, dv.upload(A, 0), dv.upload(B, 0), C, 0, kernel=kernel)
     Am = A.reshape(K, M).T if ta else A.reshape(M, K)
     Bm = B.reshape(N, K).T if tb else B.reshape(K, N)
     ref = Am @ Bm
@@ -48,6 +50,34 @@ def test_blockgemm_dense_f32(gpu, shape):
     pq.run_program(_dense_program(M, N, K), dv.upload(A, 1), dv.upload(B, 1), C, 1)
     ref = A.reshape(M, K).astype(np.float64) @ B.reshape(K, N).astype(np.float64)
     assert np.abs(dv.download(C).reshape(M, N) - ref).max() < 1e-5 * K
+
+
+def test_wavegemm_split_k_random_programs(gpu):
+    """Long pair lists (split-K with workspace reduction) vs the CPU model, rb=1,2,4."""
+    rng = np.random.default_rng(0)
+    nt = 40
+    tasks = np.zeros(nt, L.TASK_DTYPE)
+    pairs, off = [], 0
+    for i in range(nt):
+        M, N = int(rng.integers(1, 90)), int(rng.integers(1, 40))
+        npair = int(rng.integers(0, 6))
+        tasks[i] = (off, 0, M, N, M, 0, N, 1, len(pairs), len(pairs) + npair, 0, 0)
+        off += M * N
+        for _ in range(npair):
+            K = int(rng.integers(1, 3000))
+            pairs.append((int(rng.integers(0, 1000)), int(rng.integers(0, 1000)),
+                          float(rng.standard_normal()), K, M, 0, K, 1, N, 0, 1, N, 0))
+    pairs = np.array(pairs, dtype=L.PAIR_DTYPE)
+    A, B = rng.standard_normal(400000), rng.standard_normal(400000)
+    want = np.zeros(off)
+    prog = pq.Program(tasks, pairs)
+    emulate(prog, A, B, want)
+    for rb in (1, 2, 4):
+        prog = pq.Program(tasks, pairs)
+        prog.rb = rb
+        C = dv.zeros(off, 0)
+        pq.run_program(prog, dv.upload(A, 0), dv.upload(B, 0), C, 0)
+        assert np.abs(dv.download(C) - want).max() < 1e-10, rb
 
 
 def test_blockgemm_beta_accumulates(gpu):

@@ -22,10 +22,14 @@ PAIR_DTYPE = np.dtype([('a_off', '<i8'), ('b_off', '<i8'), ('scale', '<f8'), ('K
                        ('a_mlo', '<i4'), ('a_smhi', '<i4'), ('a_smlo', '<i4'), ('a_sk', '<i4'),
                        ('b_nlo', '<i4'), ('b_snhi', '<i4'), ('b_snlo', '<i4'), ('b_sk', '<i4'),
                        ('pad', '<i4')])
+UNIT_DTYPE = np.dtype([('task', '<i4'), ('m0', '<i4'), ('n0', '<i4'), ('p_begin', '<i4'),
+                       ('p_end', '<i4'), ('slot', '<i4'), ('pad0', '<i4'), ('pad1', '<i4')])
+RGROUP_DTYPE = np.dtype([('task', '<i4'), ('m0', '<i4'), ('n0', '<i4'), ('slot_begin', '<i4'),
+                         ('nslots', '<i4'), ('pad', '<i4')])
 TILE = 64
 
 EXPORTS = ['zmpo_last_error', 'zmpo_abi_version', 'zmpo_struct_sizes', 'zmpo_symAllowed',
-           'zmpo_symAllowedInt', 'zmpo_blockgemm', 'zmpo_dots', 'zmpo_dots_workspace',
+           'zmpo_symAllowedInt', 'zmpo_blockgemm', 'zmpo_wavegemm', 'zmpo_dots', 'zmpo_dots_workspace',
            'zmpo_lincomb', 'zmpo_precond', 'zmpo_gather', 'zmpo_scatter', 'zmpo_convert']
 
 
@@ -51,6 +55,7 @@ def lib():
     L.zmpo_symAllowed.argtypes = [i32, i32, vp, vp, vp] + [vp] * 8
     L.zmpo_symAllowedInt.argtypes = [i32, i32, vp, vp, vp, vp]
     L.zmpo_blockgemm.argtypes = [i32, vp, i64, vp, i64, vp, vp, vp, i32, vp]
+    L.zmpo_wavegemm.argtypes = [i32, i32, vp, vp, i64, vp, i64, vp, vp, vp, vp, vp, i32, vp]
     L.zmpo_dots.argtypes = [i32, vp, i64, i32, vp, i64, vp, vp, vp]
     L.zmpo_dots_workspace.argtypes = [i32]
     L.zmpo_dots_workspace.restype = i32

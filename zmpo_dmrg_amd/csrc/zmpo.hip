@@ -35,6 +35,8 @@ int check_hip(hipError_t e, const char* where) {
 
 static_assert(sizeof(zmpo_task) == 56, "zmpo_task layout");
 static_assert(sizeof(zmpo_pair) == 64, "zmpo_pair layout");
+static_assert(sizeof(zmpo_unit) == 32, "zmpo_unit layout");
+static_assert(sizeof(zmpo_rgroup) == 24, "zmpo_rgroup layout");
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
@@ -199,6 +201,125 @@ __global__ __launch_bounds__(NTHREADS) void blockgemm_kernel(
       }
     }
   }
+}
+
+// ---------------------------------------------------------------- wave tiles
+// Small-block variant: one wave owns a (16*RB) x 16 tile of one task and
+// walks a contiguous slice of that task's pairs, feeding operands from L1/L2
+// straight into MFMA registers (no LDS, no barriers).  RB row blocks share
+// the B fragment; two accumulator sets over alternating k-steps hide the MFMA
+// dependency latency and are added in a fixed order at the end.  A unit with
+// slot >= 0 writes its tile to the workspace; wave_reduce_kernel then adds the
+// slots of each tile in slot order (deterministic split-K).
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void wave_gemm_kernel(
+    const zmpo_task* __restrict__ tasks, const zmpo_unit* __restrict__ units, int64_t nunits,
+    const zmpo_pair* __restrict__ pairs, const T* __restrict__ A, const T* __restrict__ B,
+    T* __restrict__ C, T* __restrict__ work, int beta) {
+  typedef typename Acc<T>::type accv;
+  const int lane = threadIdx.x & 63;
+  const int64_t u = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (u >= nunits) return;  // wave-uniform; this kernel has no barriers
+  const zmpo_unit un = units[u];
+  const zmpo_task tk = tasks[un.task];
+  const int lr = lane & 15;
+  const int lk = lane >> 4;
+  accv acc0[RB], acc1[RB];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) {
+    acc0[rb] = accv{0, 0, 0, 0};
+    acc1[rb] = accv{0, 0, 0, 0};
+  }
+  for (int p = un.p_begin; p < un.p_end; ++p) {
+    const zmpo_pair pr = pairs[p];
+    const T scale = (T)pr.scale;
+    int64_t aoff[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      const int m = un.m0 + rb * 16 + lr;
+      aoff[rb] = m < tk.M ? pr.a_off + (int64_t)(m / pr.a_mlo) * pr.a_smhi +
+                                (int64_t)(m % pr.a_mlo) * pr.a_smlo
+                          : (int64_t)-1;
+    }
+    const int n = un.n0 + lr;
+    const int64_t boff = n < tk.N ? pr.b_off + (int64_t)(n / pr.b_nlo) * pr.b_snhi +
+                                        (int64_t)(n % pr.b_nlo) * pr.b_snlo
+                                  : (int64_t)-1;
+    const int K = pr.K;
+    for (int k0 = 0; k0 < K; k0 += 8) {
+      const int ka = k0 + lk;
+      const int kb = k0 + 4 + lk;
+      const T b0 = (boff >= 0 && ka < K) ? B[boff + (int64_t)ka * pr.b_sk] : (T)0;
+      const T b1 = (boff >= 0 && kb < K) ? B[boff + (int64_t)kb * pr.b_sk] : (T)0;
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        const T a0 = (aoff[rb] >= 0 && ka < K) ? A[aoff[rb] + (int64_t)ka * pr.a_sk] * scale : (T)0;
+        const T a1 = (aoff[rb] >= 0 && kb < K) ? A[aoff[rb] + (int64_t)kb * pr.a_sk] * scale : (T)0;
+        acc0[rb] = Acc<T>::mfma(a0, b0, acc0[rb]);
+        acc1[rb] = Acc<T>::mfma(a1, b1, acc1[rb]);
+      }
+    }
+  }
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const T v = acc0[rb][r] + acc1[rb][r];
+      const int row = rb * 16 + Acc<T>::row(lk, r);
+      if (un.slot >= 0) {
+        work[(int64_t)un.slot * (RB * 256) + row * 16 + lr] = v;
+      } else {
+        const int mi = un.m0 + row;
+        const int ni = un.n0 + lr;
+        if (mi < tk.M && ni < tk.N) {
+          const int64_t off = tk.c_off + (int64_t)(mi / tk.c_mlo) * tk.c_smhi +
+                              (int64_t)(mi % tk.c_mlo) * tk.c_smlo + (int64_t)ni * tk.c_sn;
+          C[off] = beta ? C[off] + v : v;
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void wave_reduce_kernel(
+    const zmpo_task* __restrict__ tasks, const zmpo_rgroup* __restrict__ groups, int64_t ngroups,
+    const T* __restrict__ work, T* __restrict__ C, int beta) {
+  const int lane = threadIdx.x & 63;
+  const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= ngroups) return;
+  const zmpo_rgroup gr = groups[g];
+  const zmpo_task tk = tasks[gr.task];
+  for (int e = lane; e < RB * 256; e += 64) {
+    const int row = e / 16, col = e % 16;
+    const int mi = gr.m0 + row, ni = gr.n0 + col;
+    if (mi >= tk.M || ni >= tk.N) continue;
+    T s = 0;
+    for (int q = 0; q < gr.nslots; ++q) s += work[(int64_t)(gr.slot_begin + q) * (RB * 256) + e];
+    const int64_t off = tk.c_off + (int64_t)(mi / tk.c_mlo) * tk.c_smhi +
+                        (int64_t)(mi % tk.c_mlo) * tk.c_smlo + (int64_t)ni * tk.c_sn;
+    C[off] = beta ? C[off] + s : s;
+  }
+}
+
+template <typename T, int RB>
+int launch_wave(const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
+                const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
+                const void* A, const void* B, void* C, void* work, int beta, hipStream_t s) {
+  if (nunits > 0) {
+    const int64_t nb = (nunits + 3) / 4;
+    hipLaunchKernelGGL((wave_gemm_kernel<T, RB>), dim3((unsigned)nb), dim3(256), 0, s, tasks,
+                       units, nunits, pairs, (const T*)A, (const T*)B, (T*)C, (T*)work, beta);
+    int rc = check_hip(hipGetLastError(), "wave_gemm launch");
+    if (rc) return rc;
+  }
+  if (ngroups > 0) {
+    const int64_t nb = (ngroups + 3) / 4;
+    hipLaunchKernelGGL((wave_reduce_kernel<T, RB>), dim3((unsigned)nb), dim3(256), 0, s, tasks,
+                       groups, ngroups, (const T*)work, (T*)C, beta);
+    return check_hip(hipGetLastError(), "wave_reduce launch");
+  }
+  return ZMPO_OK;
 }
 
 // ---------------------------------------------------------------- vectors
@@ -394,6 +515,34 @@ int zmpo_blockgemm(int dtype, const zmpo_task* tasks, int64_t ntasks, const zmpo
     case ZMPO_F32: return launch_blockgemm<float>(tasks, ntasks, pairs, ntiles, A, B, C, beta, s);
     default: return set_error(ZMPO_ERR_ARG, "blockgemm: unsupported dtype");
   }
+}
+
+int zmpo_wavegemm(int dtype, int rb, const zmpo_task* tasks, const zmpo_unit* units,
+                  int64_t nunits, const zmpo_rgroup* groups, int64_t ngroups,
+                  const zmpo_pair* pairs, const void* A, const void* B, void* C, void* work,
+                  int beta, void* stream) {
+  if (nunits > 0 && (!tasks || !units || !pairs || !A || !B || !C))
+    return set_error(ZMPO_ERR_ARG, "wavegemm: null pointer");
+  if (ngroups > 0 && (!groups || !work)) return set_error(ZMPO_ERR_ARG, "wavegemm: null workspace");
+  if ((nunits + 3) / 4 > 0x7fffffffLL || (ngroups + 3) / 4 > 0x7fffffffLL)
+    return set_error(ZMPO_ERR_ARG, "wavegemm: too many units");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == ZMPO_F64) {
+    switch (rb) {
+      case 1: return launch_wave<double, 1>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+      case 2: return launch_wave<double, 2>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+      case 4: return launch_wave<double, 4>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+      default: break;
+    }
+  } else if (dtype == ZMPO_F32) {
+    switch (rb) {
+      case 1: return launch_wave<float, 1>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+      case 2: return launch_wave<float, 2>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+      case 4: return launch_wave<float, 4>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+      default: break;
+    }
+  }
+  return set_error(ZMPO_ERR_ARG, "wavegemm: unsupported dtype or row-block count");
 }
 
 int zmpo_dots(int dtype, const void* X, int64_t ldx, int nx, const void* y, int64_t n,

@@ -438,6 +438,90 @@ def finish_tasks(tasks):
     return int(nt.sum()) if len(tasks) else 0
 
 
+def make_schedule(tasks, pairs, rb, kchunk=2048):
+    """Wave units (one (16*rb) x 16 tile x one pair chunk) and split-K groups.
+
+    Units are ordered (task, tile, chunk) so the slots of one split tile are
+    consecutive; a task whose summed K exceeds kchunk is split between pairs.
+    Pairs must be stored task by task (p_begin ascending), as lower_plan and
+    the program builders produce them.
+    """
+    nt = len(tasks)
+    M = tasks['M'].astype(np.int64)
+    N = tasks['N'].astype(np.int64)
+    pb = tasks['p_begin'].astype(np.int64)
+    pe = tasks['p_end'].astype(np.int64)
+    K = pairs['K'].astype(np.int64)
+    tm = np.maximum((M + 16 * rb - 1) // (16 * rb), 1)
+    tn = np.maximum((N + 15) // 16, 1)
+    cnt = pe - pb
+    tp = np.repeat(np.arange(nt), cnt)
+    pidx = np.concatenate([np.arange(b, e) for b, e in zip(pb[cnt > 0], pe[cnt > 0])]) \
+        if np.any(cnt > 0) else np.zeros(0, np.int64)
+    if len(pidx):
+        kk = K[pidx]
+        csum = np.cumsum(kk)
+        first = np.concatenate([[0], np.cumsum(cnt)[:-1]])[tp]
+        start = np.concatenate([[0], csum])[first]
+        cid = (csum - kk - start) // kchunk
+        new = np.ones(len(kk), bool)
+        new[1:] = (tp[1:] != tp[:-1]) | (cid[1:] != cid[:-1])
+        cst = np.flatnonzero(new)
+        cen = np.append(cst[1:], len(kk))
+        ctask = tp[cst]
+        cb, ce = pidx[cst], pidx[cen - 1] + 1
+    else:
+        ctask = np.zeros(0, np.int64)
+        cb = ce = np.zeros(0, np.int64)
+    empty = np.flatnonzero(cnt == 0)
+    ctask = np.concatenate([ctask, empty])
+    cb = np.concatenate([cb, pb[empty]])
+    ce = np.concatenate([ce, pb[empty]])
+    order = np.argsort(ctask, kind='stable')
+    ctask, cb, ce = ctask[order], cb[order], ce[order]
+    nch = np.bincount(ctask, minlength=nt)
+    cfirst = np.concatenate([[0], np.cumsum(nch)[:-1]])
+    per = tm * tn * nch
+    ut = np.repeat(np.arange(nt), per)
+    within = np.arange(int(per.sum())) - np.repeat(np.concatenate([[0], np.cumsum(per)[:-1]]), per)
+    tile = within // nch[ut]
+    chunk = within % nch[ut]
+    units = np.zeros(len(ut), dtype=_lib.UNIT_DTYPE)
+    units['task'] = ut
+    units['m0'] = (tile // tn[ut]) * 16 * rb
+    units['n0'] = (tile % tn[ut]) * 16
+    gi = cfirst[ut] + chunk
+    units['p_begin'] = cb[gi]
+    units['p_end'] = ce[gi]
+    split = nch[ut] > 1
+    slots = -np.ones(len(ut), np.int64)
+    slots[split] = np.arange(int(split.sum()))
+    units['slot'] = slots
+    first = split & (chunk == 0)
+    groups = np.zeros(int(first.sum()), dtype=_lib.RGROUP_DTYPE)
+    groups['task'] = ut[first]
+    groups['m0'] = units['m0'][first]
+    groups['n0'] = units['n0'][first]
+    groups['slot_begin'] = slots[first]
+    groups['nslots'] = nch[ut[first]]
+    return units, groups, int(split.sum())
+
+
+def choose_rb(tasks):
+    """Row blocks per wave: the largest rb whose tile padding keeps at least
+    85% of the best achievable useful fraction."""
+    M = tasks['M'].astype(np.int64)
+    N = tasks['N'].astype(np.int64)
+    useful = float(np.sum(M * N)) or 1.0
+    util = {}
+    for rb in (1, 2, 4):
+        tm = np.maximum((M + 16 * rb - 1) // (16 * rb), 1)
+        tn = np.maximum((N + 15) // 16, 1)
+        util[rb] = useful / float(np.sum(tm * 16 * rb * tn * 16))
+    best = max(util.values())
+    return max(rb for rb in util if util[rb] >= 0.85 * best)
+
+
 class Program:
     """Task/pair arrays of one grouped-GEMM launch (uploaded on first use)."""
 
@@ -447,27 +531,59 @@ class Program:
         self.ntiles = finish_tasks(tasks) if self.ntasks else 0
         self.tasks_host, self.pairs_host = tasks, pairs
         self._dev = None
+        self._sched = None
         rows = np.repeat(np.arange(self.ntasks), (tasks['p_end'] - tasks['p_begin']).astype(np.int64))
         self.flops = int(np.sum(2 * tasks['M'][rows].astype(np.int64) * tasks['N'][rows]
                                 * pairs['K'].astype(np.int64))) if self.npairs else 0
+        self.rb = choose_rb(tasks) if self.ntasks else 1
 
     def device(self):
         if self._dev is None:
             self._dev = (dv.upload_bytes(self.tasks_host), dv.upload_bytes(self.pairs_host))
         return self._dev
 
+    def schedule(self):
+        """Device units / split-K groups for the wave kernel (built once)."""
+        if self._sched is None:
+            units, groups, nslots = make_schedule(self.tasks_host, self.pairs_host, self.rb)
+            self._sched = {'units': dv.upload_bytes(units), 'nunits': len(units),
+                           'groups': dv.upload_bytes(groups), 'ngroups': len(groups),
+                           'nslots': nslots, 'units_host': units, 'groups_host': groups}
+        return self._sched
 
-def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None):
-    """Launch one grouped block GEMM: C (=|+=) sum over pairs of A_pair B_pair."""
+
+_WORK = {}
+
+
+def _workspace(n, dtype):
+    buf = _WORK.get(dtype)
+    if buf is None or buf.numel() < n:
+        buf = dv.empty(max(n, 1), dtype)
+        _WORK[dtype] = buf
+    return buf
+
+
+def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel='wave'):
+    """Launch one grouped block GEMM: C (=|+=) sum over pairs of A_pair B_pair.
+
+    kernel='wave' (default) runs one wave per small tile with deterministic
+    split-K; kernel='tile' runs the 64x64 LDS-tiled workgroup kernel."""
     if isinstance(prog, dict):
         prog = Program(prog['tasks'], prog['pairs'])
     if prog.ntasks == 0:
         return
     tasks, pairs = prog.device()
     L = _lib.lib()
-    _lib.check(L.zmpo_blockgemm(dtype, dv.ptr(tasks), prog.ntasks, dv.ptr(pairs),
-                                prog.ntiles, dv.ptr(A), dv.ptr(B), dv.ptr(C), beta,
-                                dv.stream() if stream is None else stream))
+    st = dv.stream() if stream is None else stream
+    if kernel == 'tile':
+        _lib.check(L.zmpo_blockgemm(dtype, dv.ptr(tasks), prog.ntasks, dv.ptr(pairs),
+                                    prog.ntiles, dv.ptr(A), dv.ptr(B), dv.ptr(C), beta, st))
+        return
+    sc = prog.schedule()
+    work = _workspace(sc['nslots'] * prog.rb * 256, dtype) if sc['nslots'] else None
+    _lib.check(L.zmpo_wavegemm(dtype, prog.rb, dv.ptr(tasks), dv.ptr(sc['units']), sc['nunits'],
+                               dv.ptr(sc['groups']), sc['ngroups'], dv.ptr(pairs), dv.ptr(A),
+                               dv.ptr(B), dv.ptr(C), dv.ptr(work), beta, st))
 
 
 def gather(dst, src, idx, dtype_dst=_lib.F64, dtype_src=_lib.F64):
