@@ -159,13 +159,18 @@ def main():
     ap.add_argument('--budget-gb', type=float, default=48.0)
     ap.add_argument('--cpu-ops', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--kernel', choices=['shaped', 'wave', 'tile'], default=None,
+                    help='grouped-GEMM schedule (default: library default)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one MI355X; --gpus %d is not supported (the operator-sharded '
               'multi-GPU design is documented, not built)' % args.gpus, file=sys.stderr)
         sys.exit(2)
     from zmpo_dmrg_amd import device as dv
+    from zmpo_dmrg_amd import qtensor as pq
     dv.require()
+    if args.kernel:
+        pq.KERNEL = args.kernel
     torch = dv.torch()
     dtype = 0 if args.dtype == 'f64' else 1
     mpo, isite, st, L, R, prog = build_workload(args.K, args.D, args.seed, dtype,
@@ -192,7 +197,7 @@ def main():
     nlaunch = len(prog.stages)
     avg_ms = stage_ms[dom] / nlaunch
     ach = stage_fl[dom] / nlaunch / (avg_ms * 1e-3)
-    roof = {'kernel': 'blockgemm_kernel (sigma stage %d of 4)' % (dom + 1), 'bound': 'mfma',
+    roof = {'kernel': '%s grouped GEMM (sigma stage %d of 4)' % (pq.KERNEL, dom + 1), 'bound': 'mfma',
             'achieved': ach / 1e12, 'peak': PEAK[dtype] / 1e12, 'unit': 'TFLOP/s',
             'frac': ach / PEAK[dtype], 'traffic': None,
             'launch_ms': avg_ms, 'stage_ms': [round(x, 4) for x in stage_ms],

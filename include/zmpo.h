@@ -118,6 +118,22 @@ int zmpo_wavegemm(int dtype, int rb, const zmpo_task* tasks, const zmpo_unit* un
                   const zmpo_pair* pairs, const void* A, const void* B, void* C, void* work,
                   int beta, void* stream);
 
+/* Tile shapes of zmpo_shapedgemm (rows x columns of one workgroup tile). */
+#define ZMPO_TILE_64x16 0
+#define ZMPO_TILE_128x16 1
+#define ZMPO_TILE_32x32 2
+#define ZMPO_TILE_64x32 3
+#define ZMPO_TILE_16x16 4
+#define ZMPO_TILE_64x64 5
+
+/* LDS-staged grouped GEMM with block-shaped tiles: one workgroup per
+ * zmpo_unit (unit m0/n0 step by the tile shape), same split-K groups and
+ * task/pair semantics as zmpo_wavegemm; work holds TM*TN elements per slot. */
+int zmpo_shapedgemm(int dtype, int shape, const zmpo_task* tasks, const zmpo_unit* units,
+                    int64_t nunits, const zmpo_rgroup* groups, int64_t ngroups,
+                    const zmpo_pair* pairs, const void* A, const void* B, void* C, void* work,
+                    int beta, void* stream);
+
 /* out[j] = sum_i X[j*ldx + i] * y[i] for j < nx (device out, float64;
  * work holds zmpo_dots_workspace(nx) doubles).  Fixed reduction order. */
 int zmpo_dots(int dtype, const void* X, int64_t ldx, int nx, const void* y, int64_t n,

@@ -13,8 +13,8 @@ Test infrastructure only.
 Deterministic truncation rule (shared with the device path; the reference's
 rule depends on Python-2 dict order and an unstable argsort and is not
 reproducible, mpo_dmrg_qparser.py:220-233,259): weights sorted descending;
-runs of weights equal to within 1e-10 relative (or 1e-14 absolute) form a
-tie cluster, ordered inside by (N, Sz) ascending then by position; keep the
+weights <= 1e-12 count as zero; runs of weights equal to within 1e-10
+relative form a tie cluster, ordered inside by (N, Sz) ascending then by position; keep the
 first min(Dcut, n) states; re-sort the kept states by (N, Sz), stable.  The
 reference keeps zero-weight states (thresh = -1e-14), whose eigenvectors are
 an arbitrary basis of the RDM null space; here that basis is fixed
@@ -28,6 +28,11 @@ from .mpo import build_mpo, qphys_spatial
 from .qtensor import QTensor, fromQnums
 
 
+# RDM weights (trace-normalised) at or below this are numerically zero: they
+# are ordered by QN only, and their eigenvectors get the canonical basis.
+WZERO = 1e-12
+
+
 def dpt(q1, q2):
     """Direct-product QNs, second index fastest (mpo_dmrg_qphys.py:63-68)."""
     return np.array([np.asarray(a) + np.asarray(b) for a in q1 for b in q2])
@@ -36,13 +41,14 @@ def dpt(q1, q2):
 def truncation_order(sigs, qsyms):
     """Descending weights with the deterministic tie-cluster rule."""
     n = len(sigs)
-    order = list(np.argsort(-sigs, kind='stable'))
+    w = np.where(sigs > WZERO, sigs, 0.0)
+    order = list(np.argsort(-w, kind='stable'))
     out = []
     i = 0
     while i < n:
         j = i + 1
-        w0 = sigs[order[i]]
-        while j < n and abs(sigs[order[j]] - w0) <= max(1e-10 * abs(w0), 1e-14):
+        w0 = w[order[i]]
+        while j < n and abs(w[order[j]] - w0) <= 1e-10 * abs(w0):
             j += 1
         cl = order[i:j]
         cl.sort(key=lambda k: (qsyms[k][0], qsyms[k][1], k))
@@ -95,7 +101,7 @@ def rdm_blkdiag(rho, classes, thresh, Dcut):
         rows = groups[k]
         s2, u = np.linalg.eigh(rho[np.ix_(rows, rows)])
         s2 = np.abs(s2)
-        u = canonical_null(u, s2, 1e-14)
+        u = canonical_null(u, s2, WZERO)
         uc = np.zeros((nrows, len(rows)))
         uc[rows, :] = u
         rot.append(uc)

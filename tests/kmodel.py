@@ -36,19 +36,24 @@ def emulate(prog, A, B, C, dtype=0, beta=0, stream=None):
         C[cidx] = acc if beta == 0 else C[cidx] + acc
 
 
-def emulate_wave(prog, A, B, C, dtype=0, beta=0, stream=None, kernel='wave'):
+def emulate_wave(prog, A, B, C, dtype=0, beta=0, stream=None, kernel=None):
     """Model of zmpo_wavegemm: per-unit tiles over pair chunks, split tiles
     summed per group in slot order (the kernel's own schedule)."""
     from zmpo_dmrg_amd.qtensor import Program, make_schedule
     if isinstance(prog, dict):
         prog = Program(prog['tasks'], prog['pairs'])
+    from zmpo_dmrg_amd.qtensor import KERNEL, SHAPES
+    kernel = KERNEL if kernel is None else kernel
     if kernel == 'tile':
         return emulate(prog, A, B, C, dtype, beta)
     A, B, C = _np(A), _np(B), _np(C)
     rb = prog.rb
-    units, groups, nslots = make_schedule(prog.tasks_host, prog.pairs_host, rb)
+    tile = None if kernel == 'wave' else SHAPES[prog.shape]
+    TMs, TNs = (16 * rb, 16) if tile is None else tile
+    units, groups, nslots = make_schedule(prog.tasks_host, prog.pairs_host, rb,
+                                          reuse_key=getattr(prog, 'reuse_key', None), tile=tile)
     t, p = prog.tasks_host, prog.pairs_host
-    work = np.zeros((max(nslots, 1), 16 * rb, 16))
+    work = np.zeros((max(nslots, 1), TMs, TNs))
 
     def cidx(tk, m, n):
         return (tk['c_off'] + (m // tk['c_mlo']) * tk['c_smhi']
@@ -56,10 +61,10 @@ def emulate_wave(prog, A, B, C, dtype=0, beta=0, stream=None, kernel='wave'):
 
     for u in units:
         tk = t[u['task']]
-        m = u['m0'] + np.arange(16 * rb)
-        n = u['n0'] + np.arange(16)
+        m = u['m0'] + np.arange(TMs)
+        n = u['n0'] + np.arange(TNs)
         mv, nv = m < tk['M'], n < tk['N']
-        acc = np.zeros((16 * rb, 16))
+        acc = np.zeros((TMs, TNs))
         for pi in range(int(u['p_begin']), int(u['p_end'])):
             q = p[pi]
             k = np.arange(int(q['K']))
@@ -77,8 +82,8 @@ def emulate_wave(prog, A, B, C, dtype=0, beta=0, stream=None, kernel='wave'):
     for g in groups:
         tk = t[g['task']]
         s_ = work[g['slot_begin']:g['slot_begin'] + g['nslots']].sum(0)
-        m = g['m0'] + np.arange(16 * rb)
-        n = g['n0'] + np.arange(16)
+        m = g['m0'] + np.arange(TMs)
+        n = g['n0'] + np.arange(TNs)
         mv, nv = m < tk['M'], n < tk['N']
         ci = cidx(tk, m[mv], n[nv])
         v = s_[np.ix_(mv, nv)]

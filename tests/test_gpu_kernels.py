@@ -24,7 +24,7 @@ def _dense_program(M, N, K, trans_a=False, trans_b=False):
 @pytest.mark.parametrize('shape', [(64, 64, 16), (37, 90, 33), (130, 7, 200), (1, 1, 1),
                                    (5, 9, 5000)])
 @pytest.mark.parametrize('ta,tb', [(False, False), (True, False), (False, True)])
-@pytest.mark.parametrize('kernel', ['wave', 'tile'])
+@pytest.mark.parametrize('kernel', ['wave', 'tile', 'shaped'])
 def test_blockgemm_dense_f64(gpu, shape, ta, tb, kernel):
     M, N, K = shape
     rng = np.random.default_rng(M * N + K)
@@ -72,12 +72,12 @@ def test_wavegemm_split_k_random_programs(gpu):
     want = np.zeros(off)
     prog = pq.Program(tasks, pairs)
     emulate(prog, A, B, want)
-    for rb in (1, 2, 4):
+    for kernel, rb, shape in [('wave', r, 0) for r in (1, 2, 4)] + [('shaped', 1, s) for s in pq.SHAPES]:
         prog = pq.Program(tasks, pairs)
-        prog.rb = rb
+        prog.rb, prog.shape = rb, shape
         C = dv.zeros(off, 0)
-        pq.run_program(prog, dv.upload(A, 0), dv.upload(B, 0), C, 0)
-        assert np.abs(dv.download(C) - want).max() < 1e-10, rb
+        pq.run_program(prog, dv.upload(A, 0), dv.upload(B, 0), C, 0, kernel=kernel)
+        assert np.abs(dv.download(C) - want).max() < 1e-10, (kernel, rb, shape)
 
 
 def test_blockgemm_beta_accumulates(gpu):

@@ -124,10 +124,13 @@ def test_wave_schedule_split_k_matches_direct():
     A, B = rng.standard_normal(300000), rng.standard_normal(300000)
     want = np.zeros(off)
     emulate(prog, A, B, want)
-    for rb in (1, 2, 4):
-        prog.rb = rb
-        _, groups, nslots = pq.make_schedule(tasks, pairs, rb)
-        assert nslots > 0 and len(groups) > 0
-        got = np.zeros(off)
-        emulate_wave(prog, A, B, got)
-        assert np.abs(got - want).max() < 1e-10
+    configs = [('wave', rb, 0) for rb in (1, 2, 4)] + [('shaped', 1, s) for s in pq.SHAPES]
+    for kernel, rb, shape in configs:
+        for key in (None, rng.integers(0, 3, nt)):
+            prog.rb, prog.shape, prog.reuse_key = rb, shape, key
+            tile = None if kernel == 'wave' else pq.SHAPES[shape]
+            _, groups, nslots = pq.make_schedule(tasks, pairs, rb, reuse_key=key, tile=tile)
+            assert nslots > 0 and len(groups) > 0
+            got = np.zeros(off)
+            emulate_wave(prog, A, B, got, kernel=kernel)
+            assert np.abs(got - want).max() < 1e-10

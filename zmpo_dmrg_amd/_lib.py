@@ -29,7 +29,7 @@ RGROUP_DTYPE = np.dtype([('task', '<i4'), ('m0', '<i4'), ('n0', '<i4'), ('slot_b
 TILE = 64
 
 EXPORTS = ['zmpo_last_error', 'zmpo_abi_version', 'zmpo_struct_sizes', 'zmpo_symAllowed',
-           'zmpo_symAllowedInt', 'zmpo_blockgemm', 'zmpo_wavegemm', 'zmpo_dots', 'zmpo_dots_workspace',
+           'zmpo_symAllowedInt', 'zmpo_blockgemm', 'zmpo_wavegemm', 'zmpo_shapedgemm', 'zmpo_dots', 'zmpo_dots_workspace',
            'zmpo_lincomb', 'zmpo_precond', 'zmpo_gather', 'zmpo_scatter', 'zmpo_convert']
 
 
@@ -56,6 +56,7 @@ def lib():
     L.zmpo_symAllowedInt.argtypes = [i32, i32, vp, vp, vp, vp]
     L.zmpo_blockgemm.argtypes = [i32, vp, i64, vp, i64, vp, vp, vp, i32, vp]
     L.zmpo_wavegemm.argtypes = [i32, i32, vp, vp, i64, vp, i64, vp, vp, vp, vp, vp, i32, vp]
+    L.zmpo_shapedgemm.argtypes = [i32, i32, vp, vp, i64, vp, i64, vp, vp, vp, vp, vp, i32, vp]
     L.zmpo_dots.argtypes = [i32, vp, i64, i32, vp, i64, vp, vp, vp]
     L.zmpo_dots_workspace.argtypes = [i32]
     L.zmpo_dots_workspace.restype = i32

@@ -206,31 +206,47 @@ __global__ __launch_bounds__(NTHREADS) void blockgemm_kernel(
 // ---------------------------------------------------------------- wave tiles
 // Small-block variant: one wave owns a (16*RB) x 16 tile of one task and
 // walks a contiguous slice of that task's pairs, feeding operands from L1/L2
-// straight into MFMA registers (no LDS, no barriers).  RB row blocks share
-// the B fragment; two accumulator sets over alternating k-steps hide the MFMA
-// dependency latency and are added in a fixed order at the end.  A unit with
+// straight into MFMA registers (LDS only stages the output tile).  RB row blocks share
+// the B fragment; four accumulator chains over consecutive k-steps hide the
+// MFMA dependency latency and are added in a fixed order at the end.  A unit with
 // slot >= 0 writes its tile to the workspace; wave_reduce_kernel then adds the
 // slots of each tile in slot order (deterministic split-K).
+// Workgroups are handed to the 8 XCDs round-robin; map hardware block b to
+// logical block so that each XCD runs one contiguous range of units (tasks
+// that share operand blocks are adjacent, so their re-reads hit one L2).
+__device__ inline int64_t xcd_logical_block(int64_t b, int64_t nb) {
+  const int64_t x = b % 8, i = b / 8;
+  return x * (nb / 8) + (x < nb % 8 ? x : nb % 8) + i;
+}
+
 template <typename T, int RB>
 __global__ __launch_bounds__(256) void wave_gemm_kernel(
     const zmpo_task* __restrict__ tasks, const zmpo_unit* __restrict__ units, int64_t nunits,
     const zmpo_pair* __restrict__ pairs, const T* __restrict__ A, const T* __restrict__ B,
     T* __restrict__ C, T* __restrict__ work, int beta) {
   typedef typename Acc<T>::type accv;
+  __shared__ T stage[4][RB * 256];
   const int lane = threadIdx.x & 63;
-  const int64_t u = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (u >= nunits) return;  // wave-uniform; this kernel has no barriers
-  const zmpo_unit un = units[u];
+  const int wv = threadIdx.x >> 6;
+  const int64_t blk = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int64_t u = blk * 4 + wv;
+  const bool active = u < nunits;  // wave-uniform
+  const zmpo_unit un = units[active ? u : 0];
   const zmpo_task tk = tasks[un.task];
   const int lr = lane & 15;
   const int lk = lane >> 4;
-  accv acc0[RB], acc1[RB];
+  // each row's 16 consecutive k values (one 128-byte line in FP64) are loaded
+  // per iteration as four k-steps; NCH accumulator chains (4 for one row
+  // block, 2 otherwise -- the row blocks already give independent MFMAs) keep
+  // the register budget at >= 2 waves per SIMD
+  constexpr int NCH = RB == 1 ? 4 : 2;
+  accv acc[NCH][RB];
 #pragma unroll
-  for (int rb = 0; rb < RB; ++rb) {
-    acc0[rb] = accv{0, 0, 0, 0};
-    acc1[rb] = accv{0, 0, 0, 0};
-  }
-  for (int p = un.p_begin; p < un.p_end; ++p) {
+  for (int c = 0; c < NCH; ++c)
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) acc[c][rb] = accv{0, 0, 0, 0};
+  const int p_end = active ? un.p_end : un.p_begin;
+  for (int p = un.p_begin; p < p_end; ++p) {
     const zmpo_pair pr = pairs[p];
     const T scale = (T)pr.scale;
     int64_t aoff[RB];
@@ -246,38 +262,52 @@ __global__ __launch_bounds__(256) void wave_gemm_kernel(
                                         (int64_t)(n % pr.b_nlo) * pr.b_snlo
                                   : (int64_t)-1;
     const int K = pr.K;
-    for (int k0 = 0; k0 < K; k0 += 8) {
-      const int ka = k0 + lk;
-      const int kb = k0 + 4 + lk;
-      const T b0 = (boff >= 0 && ka < K) ? B[boff + (int64_t)ka * pr.b_sk] : (T)0;
-      const T b1 = (boff >= 0 && kb < K) ? B[boff + (int64_t)kb * pr.b_sk] : (T)0;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      T bv[4];
+      T av[4][RB];
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb) {
-        const T a0 = (aoff[rb] >= 0 && ka < K) ? A[aoff[rb] + (int64_t)ka * pr.a_sk] * scale : (T)0;
-        const T a1 = (aoff[rb] >= 0 && kb < K) ? A[aoff[rb] + (int64_t)kb * pr.a_sk] * scale : (T)0;
-        acc0[rb] = Acc<T>::mfma(a0, b0, acc0[rb]);
-        acc1[rb] = Acc<T>::mfma(a1, b1, acc1[rb]);
+      for (int c = 0; c < 4; ++c) {
+        const int k = k0 + 4 * c + lk;
+        bv[c] = (boff >= 0 && k < K) ? B[boff + (int64_t)k * pr.b_sk] : (T)0;
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+          av[c][rb] = (aoff[rb] >= 0 && k < K) ? A[aoff[rb] + (int64_t)k * pr.a_sk] * scale : (T)0;
       }
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+          acc[c % NCH][rb] = Acc<T>::mfma(av[c][rb], bv[c], acc[c % NCH][rb]);
     }
   }
+  // Stage the tile in this wave's LDS slice, then write it out row-major so
+  // that consecutive lanes store consecutive addresses of each output row.
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const T v = acc0[rb][r] + acc1[rb][r];
-      const int row = rb * 16 + Acc<T>::row(lk, r);
-      if (un.slot >= 0) {
-        work[(int64_t)un.slot * (RB * 256) + row * 16 + lr] = v;
-      } else {
-        const int mi = un.m0 + row;
-        const int ni = un.n0 + lr;
-        if (mi < tk.M && ni < tk.N) {
-          const int64_t off = tk.c_off + (int64_t)(mi / tk.c_mlo) * tk.c_smhi +
-                              (int64_t)(mi % tk.c_mlo) * tk.c_smlo + (int64_t)ni * tk.c_sn;
-          C[off] = beta ? C[off] + v : v;
-        }
-      }
+      T v = acc[0][rb][r];
+#pragma unroll
+      for (int c = 1; c < NCH; ++c) v += acc[c][rb][r];
+      stage[wv][(rb * 16 + Acc<T>::row(lk, r)) * 16 + lr] = v;
     }
+  }
+  __syncthreads();
+  if (!active) return;
+  if (un.slot >= 0) {
+    T* __restrict__ w = work + (int64_t)un.slot * (RB * 256);
+    for (int e = lane; e < RB * 256; e += 64) w[e] = stage[wv][e];
+    return;
+  }
+  const int mv = min(16 * RB, tk.M - un.m0);
+  const int nv = min(16, tk.N - un.n0);
+  for (int e = lane; e < mv * nv; e += 64) {
+    const int row = e / nv, col = e % nv;
+    const int mi = un.m0 + row, ni = un.n0 + col;
+    const int64_t off = tk.c_off + (int64_t)(mi / tk.c_mlo) * tk.c_smhi +
+                        (int64_t)(mi % tk.c_mlo) * tk.c_smlo + (int64_t)ni * tk.c_sn;
+    const T v = stage[wv][row * 16 + col];
+    C[off] = beta ? C[off] + v : v;
   }
 }
 
@@ -320,6 +350,216 @@ int launch_wave(const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
     return check_hip(hipGetLastError(), "wave_reduce launch");
   }
   return ZMPO_OK;
+}
+
+// ---------------------------------------------------------------- shaped tiles
+// LDS-staged variant with block-shaped tiles: one workgroup (4 waves) owns a
+// TMxTN tile of one task and a slice of its pairs (the zmpo_unit schedule of
+// zmpo_wavegemm).  Operand tiles are copied to LDS with consecutive threads
+// on the contiguous direction of each operand (one 128-byte line per row for
+// K-contiguous operands), double-buffered over k, and the MFMA 16x16 blocks
+// are spread over the waves.
+// Per-thread load state for the operand tiles of one pair.  Element e of a
+// KTxTM (or KTxTN) tile is owned by thread e % 256; its (row, k) split
+// depends on which operand direction is contiguous, so that consecutive
+// threads read consecutive addresses.
+template <typename T, int ROWS, int KT>
+struct TileLoader {
+  static constexpr int NE = ROWS * KT / 256;
+  int64_t roff[NE];  // row offset (relative to base), -1 = padding row
+  int rr[NE], kk[NE];
+  const T* base;
+  int64_t ks;
+  int K;
+  T scale;
+  __device__ __forceinline__ void setup(int tid, const T* b, int64_t off, int K_, int64_t sk, int mlo,
+                        int64_t shi, int64_t slo, int r0, int rmax, T sc) {
+    base = b + off;
+    ks = sk;
+    K = K_;
+    scale = sc;
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      const int e = tid + 256 * i;
+      const bool kmaj = sk == 1;
+      kk[i] = kmaj ? e % KT : e / ROWS;
+      rr[i] = kmaj ? e / KT : e % ROWS;
+      const int r = r0 + rr[i];
+      roff[i] = r < rmax ? (int64_t)(r / mlo) * shi + (int64_t)(r % mlo) * slo : (int64_t)-1;
+    }
+  }
+  __device__ __forceinline__ void load(T (*dst)[ROWS + 1], int k0) const {
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      const int k = k0 + kk[i];
+      dst[kk[i]][rr[i]] = (roff[i] >= 0 && k < K) ? base[roff[i] + (int64_t)k * ks] * scale : (T)0;
+    }
+  }
+};
+
+// One workgroup = one unit; its pairs' k-tiles form one flat, double-buffered
+// pipeline (the next tile -- possibly of the next pair -- is loaded while the
+// current one feeds the MFMAs), so a unit pays the memory latency about once.
+template <typename T, int TMv, int TNv>
+__global__ __launch_bounds__(256) void shaped_gemm_kernel(
+    const zmpo_task* __restrict__ tasks, const zmpo_unit* __restrict__ units, int64_t nunits,
+    const zmpo_pair* __restrict__ pairs, const T* __restrict__ A, const T* __restrict__ B,
+    T* __restrict__ C, T* __restrict__ work, int beta) {
+  typedef typename Acc<T>::type accv;
+  constexpr int KT = 16;
+  constexpr int BM = TMv / 16, BN = TNv / 16;
+  constexpr int NBLK = BM * BN;
+  constexpr int PER = (NBLK + 3) / 4;
+  __shared__ T As[2][KT][TMv + 1];
+  __shared__ T Bs[2][KT][TNv + 1];
+  __shared__ T stage[TMv * TNv];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t u = xcd_logical_block(blockIdx.x, gridDim.x);
+  if (u >= nunits) return;  // workgroup-uniform
+  const zmpo_unit un = units[u];
+  const zmpo_task tk = tasks[un.task];
+  accv acc[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) acc[i] = accv{0, 0, 0, 0};
+  TileLoader<T, TMv, KT> la;
+  TileLoader<T, TNv, KT> lb;
+  // load cursor (pair, k-tile)
+  int lp = un.p_begin, lkt = 0, lK = 0;
+#define ZMPO_SETUP_PAIR(P)                                                                     \
+  {                                                                                              \
+    const zmpo_pair pr = pairs[(P)];                                                             \
+    la.setup(tid, A, pr.a_off, pr.K, pr.a_sk, pr.a_mlo, pr.a_smhi, pr.a_smlo, un.m0, tk.M,      \
+             (T)pr.scale);                                                                       \
+    lb.setup(tid, B, pr.b_off, pr.K, pr.b_sk, pr.b_nlo, pr.b_snhi, pr.b_snlo, un.n0, tk.N, (T)1); \
+    lK = pr.K;                                                                                   \
+  }
+  int ks_cur = 0;  // k-steps held by the buffer being computed (0 = done)
+  if (lp < un.p_end) {
+    ZMPO_SETUP_PAIR(lp);
+    la.load(As[0], 0);
+    lb.load(Bs[0], 0);
+    ks_cur = min(KT, lK);
+    lkt = 1;
+  }
+  __syncthreads();
+  int buf = 0;
+  while (ks_cur > 0) {
+    // advance the load cursor and prefetch into the other buffer
+    int ks_nxt = 0;
+    if (lkt * KT >= lK) {
+      ++lp;
+      lkt = 0;
+      if (lp < un.p_end) ZMPO_SETUP_PAIR(lp);
+    }
+    if (lp < un.p_end) {
+      la.load(As[buf ^ 1], lkt * KT);
+      lb.load(Bs[buf ^ 1], lkt * KT);
+      ks_nxt = min(KT, lK - lkt * KT);
+      ++lkt;
+    }
+#pragma unroll
+    for (int kk = 0; kk < KT; kk += 4) {
+      if (kk >= ks_cur) break;  // uniform
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int blkid = wv + 4 * i;
+        if (blkid < NBLK) {
+          const int bm = blkid % BM, bn = blkid / BM;
+          acc[i] = Acc<T>::mfma(As[buf][kk + lk][bm * 16 + lr], Bs[buf][kk + lk][bn * 16 + lr],
+                                acc[i]);
+        }
+      }
+    }
+    __syncthreads();
+    buf ^= 1;
+    ks_cur = ks_nxt;
+  }
+#undef ZMPO_SETUP_PAIR
+  // stage the tile, then write it out row-major (coalesced rows)
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int blkid = wv + 4 * i;
+    if (blkid < NBLK) {
+      const int bm = blkid % BM, bn = blkid / BM;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        stage[(bm * 16 + Acc<T>::row(lk, r)) * TNv + bn * 16 + lr] = acc[i][r];
+    }
+  }
+  __syncthreads();
+  if (un.slot >= 0) {
+    T* __restrict__ w = work + (int64_t)un.slot * (TMv * TNv);
+    for (int e = tid; e < TMv * TNv; e += 256) w[e] = stage[e];
+    return;
+  }
+  const int mv = min(TMv, tk.M - un.m0);
+  const int nv = min(TNv, tk.N - un.n0);
+  for (int e = tid; e < mv * nv; e += 256) {
+    const int row = e / nv, col = e % nv;
+    const int mi = un.m0 + row, ni = un.n0 + col;
+    const int64_t off = tk.c_off + (int64_t)(mi / tk.c_mlo) * tk.c_smhi +
+                        (int64_t)(mi % tk.c_mlo) * tk.c_smlo + (int64_t)ni * tk.c_sn;
+    const T v = stage[row * TNv + col];
+    C[off] = beta ? C[off] + v : v;
+  }
+}
+
+template <typename T, int TMv, int TNv>
+__global__ __launch_bounds__(256) void shaped_reduce_kernel(
+    const zmpo_task* __restrict__ tasks, const zmpo_rgroup* __restrict__ groups, int64_t ngroups,
+    const T* __restrict__ work, T* __restrict__ C, int beta) {
+  const int64_t g = blockIdx.x;
+  if (g >= ngroups) return;
+  const zmpo_rgroup gr = groups[g];
+  const zmpo_task tk = tasks[gr.task];
+  const int mv = min(TMv, tk.M - gr.m0);
+  const int nv = min(TNv, tk.N - gr.n0);
+  for (int e = threadIdx.x; e < mv * nv; e += blockDim.x) {
+    const int row = e / nv, col = e % nv;
+    T s = 0;
+    for (int q = 0; q < gr.nslots; ++q)
+      s += work[(int64_t)(gr.slot_begin + q) * (TMv * TNv) + row * TNv + col];
+    const int mi = gr.m0 + row, ni = gr.n0 + col;
+    const int64_t off = tk.c_off + (int64_t)(mi / tk.c_mlo) * tk.c_smhi +
+                        (int64_t)(mi % tk.c_mlo) * tk.c_smlo + (int64_t)ni * tk.c_sn;
+    C[off] = beta ? C[off] + s : s;
+  }
+}
+
+template <typename T, int TMv, int TNv>
+int launch_shaped(const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
+                  const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
+                  const void* A, const void* B, void* C, void* work, int beta, hipStream_t s) {
+  if (nunits > 0) {
+    hipLaunchKernelGGL((shaped_gemm_kernel<T, TMv, TNv>), dim3((unsigned)nunits), dim3(256), 0, s,
+                       tasks, units, nunits, pairs, (const T*)A, (const T*)B, (T*)C, (T*)work,
+                       beta);
+    int rc = check_hip(hipGetLastError(), "shaped_gemm launch");
+    if (rc) return rc;
+  }
+  if (ngroups > 0) {
+    hipLaunchKernelGGL((shaped_reduce_kernel<T, TMv, TNv>), dim3((unsigned)ngroups), dim3(256), 0,
+                       s, tasks, groups, ngroups, (const T*)work, (T*)C, beta);
+    return check_hip(hipGetLastError(), "shaped_reduce launch");
+  }
+  return ZMPO_OK;
+}
+
+template <typename T>
+int dispatch_shaped(int shape, const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
+                    const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
+                    const void* A, const void* B, void* C, void* work, int beta, hipStream_t s) {
+  switch (shape) {
+    case ZMPO_TILE_64x16: return launch_shaped<T, 64, 16>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_128x16: return launch_shaped<T, 128, 16>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_32x32: return launch_shaped<T, 32, 32>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_64x32: return launch_shaped<T, 64, 32>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_16x16: return launch_shaped<T, 16, 16>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_64x64: return launch_shaped<T, 64, 64>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    default: return set_error(ZMPO_ERR_ARG, "shapedgemm: unknown tile shape");
+  }
 }
 
 // ---------------------------------------------------------------- vectors
@@ -543,6 +783,23 @@ int zmpo_wavegemm(int dtype, int rb, const zmpo_task* tasks, const zmpo_unit* un
     }
   }
   return set_error(ZMPO_ERR_ARG, "wavegemm: unsupported dtype or row-block count");
+}
+
+int zmpo_shapedgemm(int dtype, int shape, const zmpo_task* tasks, const zmpo_unit* units,
+                    int64_t nunits, const zmpo_rgroup* groups, int64_t ngroups,
+                    const zmpo_pair* pairs, const void* A, const void* B, void* C, void* work,
+                    int beta, void* stream) {
+  if (nunits > 0 && (!tasks || !units || !pairs || !A || !B || !C))
+    return set_error(ZMPO_ERR_ARG, "shapedgemm: null pointer");
+  if (ngroups > 0 && (!groups || !work)) return set_error(ZMPO_ERR_ARG, "shapedgemm: null workspace");
+  if (nunits > 0x7fffffffLL || ngroups > 0x7fffffffLL)
+    return set_error(ZMPO_ERR_ARG, "shapedgemm: too many units");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == ZMPO_F64)
+    return dispatch_shaped<double>(shape, tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+  if (dtype == ZMPO_F32)
+    return dispatch_shaped<float>(shape, tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+  return set_error(ZMPO_ERR_ARG, "shapedgemm: unsupported dtype");
 }
 
 int zmpo_dots(int dtype, const void* X, int64_t ldx, int nx, const void* y, int64_t n,

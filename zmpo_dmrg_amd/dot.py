@@ -8,8 +8,8 @@ setupQtmpCI :192-203, genHDiag :206-231, finalizeDot :259-285, decimation
 Sigma, diagonal and the Davidson vector algebra run on the device.  The
 reduced-density-matrix decimation runs on the host with numpy over the QN
 sectors (per-sector eigh, as mpo_dmrg_qparser.rdm_blkdiag); its truncation
-rule is deterministic: weights sorted descending; runs of weights equal to
-1e-10 relative (1e-14 absolute) form tie clusters ordered by (N, Sz) then
+rule is deterministic: weights sorted descending, weights <= 1e-12 counting as
+zero; runs of weights equal to 1e-10 relative form tie clusters ordered by (N, Sz) then
 position; keep the first min(Dcut, n); re-sort the kept states by (N, Sz).
 Zero-weight states (kept by the reference, thresh = -1e-14) get a fixed
 basis of the RDM null space (canonical_null) instead of LAPACK's arbitrary
@@ -26,6 +26,11 @@ from .pdvdson import EigenSolver
 from .qtensor import QTensor, fromQnums
 
 
+# RDM weights (trace-normalised) at or below this are numerically zero: they
+# are ordered by QN only, and their eigenvectors get the canonical basis.
+WZERO = 1e-12
+
+
 def dpt(q1, q2):
     """Direct-product QNs, second index fastest (mpo_dmrg_qphys.py:63-68)."""
     q1, q2 = np.asarray(q1), np.asarray(q2)
@@ -34,13 +39,14 @@ def dpt(q1, q2):
 
 def truncation_order(sigs, qsyms):
     n = len(sigs)
-    order = list(np.argsort(-sigs, kind='stable'))
+    w = np.where(sigs > WZERO, sigs, 0.0)
+    order = list(np.argsort(-w, kind='stable'))
     out = []
     i = 0
     while i < n:
         j = i + 1
-        w0 = sigs[order[i]]
-        while j < n and abs(sigs[order[j]] - w0) <= max(1e-10 * abs(w0), 1e-14):
+        w0 = w[order[i]]
+        while j < n and abs(w[order[j]] - w0) <= 1e-10 * abs(w0):
             j += 1
         cl = order[i:j]
         cl.sort(key=lambda k: (qsyms[k][0], qsyms[k][1], k))
@@ -189,7 +195,7 @@ def rdm_blkdiag(mats, classes, thresh, Dcut, side):
             rho = rho / tr
         s2, u = np.linalg.eigh(rho)
         s2 = np.abs(s2)
-        u = canonical_null(u, s2, 1e-14)
+        u = canonical_null(u, s2, WZERO)
         for c in range(len(rows)):
             rot_cols.append((rows, u[:, c]))
         sigs.append(s2)

@@ -209,7 +209,8 @@ class SigmaProgram:
                 t, pr = _concat(parts[k])
                 progs.append(Program(t, pr))
             t, pr = _merge_shared(parts[3], qtmp.nblks_allowed) if parts[3] else _concat([])
-            progs.append(Program(t, pr))
+            # sigma blocks with the same right sector read the same R blocks
+            progs.append(Program(t, pr, reuse_key=qtmp.blk_isyms[:, 3] if len(t) else None))
             for k in range(4):
                 self.flops_stage[k] += progs[k].flops
             self.stages.append(progs)

@@ -18,6 +18,8 @@ ascending block index.  Two differences, both internal to this package:
 block order, same internal-tuple order, qtensor.py:563-637) with integer
 numpy, and lowers it to GEMM tasks (``lower_plan``).
 """
+import os
+
 import numpy as np
 
 from . import _lib
@@ -438,11 +440,16 @@ def finish_tasks(tasks):
     return int(nt.sum()) if len(tasks) else 0
 
 
-def make_schedule(tasks, pairs, rb, kchunk=2048):
-    """Wave units (one (16*rb) x 16 tile x one pair chunk) and split-K groups.
+def make_schedule(tasks, pairs, rb, kchunk=2048, reuse_key=None, tile=None):
+    """Units (one output tile x one pair chunk) and split-K groups.
 
-    Units are ordered (task, tile, chunk) so the slots of one split tile are
-    consecutive; a task whose summed K exceeds kchunk is split between pairs.
+    The tile is (16*rb) x 16 for the wave kernel, or tile=(TM, TN) for the
+    shaped kernel.
+    Slots of one split tile are consecutive (task, tile, chunk order); a task
+    whose summed K exceeds kchunk is split between pairs.  With ``reuse_key``
+    (one int per task: tasks with equal keys read the same B blocks in the
+    same pair order) the execution order becomes (key, chunk, task, tile), so
+    co-resident waves stream the same B slice through one L2.
     Pairs must be stored task by task (p_begin ascending), as lower_plan and
     the program builders produce them.
     """
@@ -452,8 +459,9 @@ def make_schedule(tasks, pairs, rb, kchunk=2048):
     pb = tasks['p_begin'].astype(np.int64)
     pe = tasks['p_end'].astype(np.int64)
     K = pairs['K'].astype(np.int64)
-    tm = np.maximum((M + 16 * rb - 1) // (16 * rb), 1)
-    tn = np.maximum((N + 15) // 16, 1)
+    TMs, TNs = (16 * rb, 16) if tile is None else tile
+    tm = np.maximum((M + TMs - 1) // TMs, 1)
+    tn = np.maximum((N + TNs - 1) // TNs, 1)
     cnt = pe - pb
     tp = np.repeat(np.arange(nt), cnt)
     pidx = np.concatenate([np.arange(b, e) for b, e in zip(pb[cnt > 0], pe[cnt > 0])]) \
@@ -484,12 +492,12 @@ def make_schedule(tasks, pairs, rb, kchunk=2048):
     per = tm * tn * nch
     ut = np.repeat(np.arange(nt), per)
     within = np.arange(int(per.sum())) - np.repeat(np.concatenate([[0], np.cumsum(per)[:-1]]), per)
-    tile = within // nch[ut]
+    tilei = within // nch[ut]
     chunk = within % nch[ut]
     units = np.zeros(len(ut), dtype=_lib.UNIT_DTYPE)
     units['task'] = ut
-    units['m0'] = (tile // tn[ut]) * 16 * rb
-    units['n0'] = (tile % tn[ut]) * 16
+    units['m0'] = (tilei // tn[ut]) * TMs
+    units['n0'] = (tilei % tn[ut]) * TNs
     gi = cfirst[ut] + chunk
     units['p_begin'] = cb[gi]
     units['p_end'] = ce[gi]
@@ -498,13 +506,41 @@ def make_schedule(tasks, pairs, rb, kchunk=2048):
     slots[split] = np.arange(int(split.sum()))
     units['slot'] = slots
     first = split & (chunk == 0)
-    groups = np.zeros(int(first.sum()), dtype=_lib.RGROUP_DTYPE)
-    groups['task'] = ut[first]
-    groups['m0'] = units['m0'][first]
-    groups['n0'] = units['n0'][first]
-    groups['slot_begin'] = slots[first]
-    groups['nslots'] = nch[ut[first]]
+    if reuse_key is not None:
+        key = np.asarray(reuse_key, dtype=np.int64)[ut]
+        order = np.lexsort((tilei, ut, chunk, key))
+        units = units[order]
+        first_units = units[np.flatnonzero(first[order])]
+    else:
+        first_units = units[np.flatnonzero(first)]
+    groups = np.zeros(len(first_units), dtype=_lib.RGROUP_DTYPE)
+    groups['task'] = first_units['task']
+    groups['m0'] = first_units['m0']
+    groups['n0'] = first_units['n0']
+    groups['slot_begin'] = first_units['slot']
+    groups['nslots'] = nch[first_units['task']]
     return units, groups, int(split.sum())
+
+
+SHAPES = {0: (64, 16), 1: (128, 16), 2: (32, 32), 3: (64, 32), 4: (16, 16), 5: (64, 64)}
+
+
+def choose_shape(tasks, pairs):
+    """Tile shape for the shaped kernel: minimise the modelled cost
+    sum over units of (TM*TN*Kpad + per-unit overhead), Kpad = K rounded to 16."""
+    M = tasks['M'].astype(np.int64)
+    N = tasks['N'].astype(np.int64)
+    cnt = (tasks['p_end'] - tasks['p_begin']).astype(np.int64)
+    rows = np.repeat(np.arange(len(tasks)), cnt)
+    kpad = np.bincount(rows, weights=((pairs['K'].astype(np.int64) + 15) // 16) * 16,
+                       minlength=len(tasks)) if len(pairs) else np.zeros(len(tasks))
+    best, cost_best = 0, None
+    for sid, (tm_, tn_) in SHAPES.items():
+        ntile = np.maximum((M + tm_ - 1) // tm_, 1) * np.maximum((N + tn_ - 1) // tn_, 1)
+        cost = float(np.sum(ntile * (tm_ * tn_ * np.maximum(kpad, 16) + 4096)))
+        if cost_best is None or cost < cost_best:
+            best, cost_best = sid, cost
+    return best
 
 
 def choose_rb(tasks):
@@ -525,7 +561,8 @@ def choose_rb(tasks):
 class Program:
     """Task/pair arrays of one grouped-GEMM launch (uploaded on first use)."""
 
-    def __init__(self, tasks, pairs):
+    def __init__(self, tasks, pairs, reuse_key=None):
+        self.reuse_key = reuse_key
         self.ntasks = len(tasks)
         self.npairs = len(pairs)
         self.ntiles = finish_tasks(tasks) if self.ntasks else 0
@@ -536,20 +573,27 @@ class Program:
         self.flops = int(np.sum(2 * tasks['M'][rows].astype(np.int64) * tasks['N'][rows]
                                 * pairs['K'].astype(np.int64))) if self.npairs else 0
         self.rb = choose_rb(tasks) if self.ntasks else 1
+        self.shape = choose_shape(tasks, pairs) if self.ntasks else 0
 
     def device(self):
         if self._dev is None:
             self._dev = (dv.upload_bytes(self.tasks_host), dv.upload_bytes(self.pairs_host))
         return self._dev
 
-    def schedule(self):
-        """Device units / split-K groups for the wave kernel (built once)."""
+    def schedule(self, kernel='wave'):
+        """Device units / split-K groups for the wave or shaped kernel (built once each)."""
         if self._sched is None:
-            units, groups, nslots = make_schedule(self.tasks_host, self.pairs_host, self.rb)
-            self._sched = {'units': dv.upload_bytes(units), 'nunits': len(units),
-                           'groups': dv.upload_bytes(groups), 'ngroups': len(groups),
-                           'nslots': nslots, 'units_host': units, 'groups_host': groups}
-        return self._sched
+            self._sched = {}
+        if kernel not in self._sched:
+            tile = None if kernel == 'wave' else SHAPES[self.shape]
+            units, groups, nslots = make_schedule(self.tasks_host, self.pairs_host, self.rb,
+                                                  reuse_key=self.reuse_key, tile=tile)
+            size = self.rb * 256 if kernel == 'wave' else tile[0] * tile[1]
+            self._sched[kernel] = {'units': dv.upload_bytes(units), 'nunits': len(units),
+                                   'groups': dv.upload_bytes(groups), 'ngroups': len(groups),
+                                   'nslots': nslots, 'slot_size': size,
+                                   'units_host': units, 'groups_host': groups}
+        return self._sched[kernel]
 
 
 _WORK = {}
@@ -563,11 +607,16 @@ def _workspace(n, dtype):
     return buf
 
 
-def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel='wave'):
+KERNEL = os.environ.get('ZMPO_KERNEL', 'shaped')
+
+
+def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None):
     """Launch one grouped block GEMM: C (=|+=) sum over pairs of A_pair B_pair.
 
-    kernel='wave' (default) runs one wave per small tile with deterministic
-    split-K; kernel='tile' runs the 64x64 LDS-tiled workgroup kernel."""
+    kernel='shaped' (default) runs LDS-staged block-shaped workgroup tiles,
+    'wave' one wave per small tile, both with deterministic split-K;
+    'tile' runs the 64x64 LDS-tiled workgroup kernel."""
+    kernel = KERNEL if kernel is None else kernel
     if isinstance(prog, dict):
         prog = Program(prog['tasks'], prog['pairs'])
     if prog.ntasks == 0:
@@ -579,11 +628,18 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel='wave
         _lib.check(L.zmpo_blockgemm(dtype, dv.ptr(tasks), prog.ntasks, dv.ptr(pairs),
                                     prog.ntiles, dv.ptr(A), dv.ptr(B), dv.ptr(C), beta, st))
         return
-    sc = prog.schedule()
-    work = _workspace(sc['nslots'] * prog.rb * 256, dtype) if sc['nslots'] else None
-    _lib.check(L.zmpo_wavegemm(dtype, prog.rb, dv.ptr(tasks), dv.ptr(sc['units']), sc['nunits'],
-                               dv.ptr(sc['groups']), sc['ngroups'], dv.ptr(pairs), dv.ptr(A),
-                               dv.ptr(B), dv.ptr(C), dv.ptr(work), beta, st))
+    sc = prog.schedule(kernel)
+    work = _workspace(sc['nslots'] * sc['slot_size'], dtype) if sc['nslots'] else None
+    if kernel == 'wave':
+        _lib.check(L.zmpo_wavegemm(dtype, prog.rb, dv.ptr(tasks), dv.ptr(sc['units']),
+                                   sc['nunits'], dv.ptr(sc['groups']), sc['ngroups'],
+                                   dv.ptr(pairs), dv.ptr(A), dv.ptr(B), dv.ptr(C), dv.ptr(work),
+                                   beta, st))
+    else:
+        _lib.check(L.zmpo_shapedgemm(dtype, prog.shape, dv.ptr(tasks), dv.ptr(sc['units']),
+                                     sc['nunits'], dv.ptr(sc['groups']), sc['ngroups'],
+                                     dv.ptr(pairs), dv.ptr(A), dv.ptr(B), dv.ptr(C),
+                                     dv.ptr(work), beta, st))
 
 
 def gather(dst, src, idx, dtype_dst=_lib.F64, dtype_src=_lib.F64):
