@@ -28,7 +28,7 @@ def log(msg):
     print('[bench] ' + msg, file=sys.stderr, flush=True)
 
 
-def build_workload(K, D, seed, dtype, budget):
+def build_workload(K, D, seed, dtype, budget, fused=True):
     from zmpo_dmrg_amd import device as dv
     from zmpo_dmrg_amd import kernelQt as kq
     from zmpo_dmrg_amd import mpo as pmpo
@@ -71,37 +71,71 @@ def build_workload(K, D, seed, dtype, budget):
     st.qlst = [ql[isite], pmpo.qphys(), pmpo.qphys(), qr[isite + 2]]
     setupQtmpCI(st, (float(K), 0.0))
     t = time.time()
-    prog = kq.SigmaProgram(mpo, isite, st.qtmp, st.qkey, L, R, mpo.hpwts, budget=budget)
+    prog = kq.SigmaProgram(mpo, isite, st.qtmp, st.qkey, L, R, mpo.hpwts, budget=budget,
+                           fused=fused)
     log('sigma plan: ndim0=%d flops=%.3e chunks=%d (%.1fs)' % (
         st.qtmp.size_allowed, prog.flops, len(prog.chunks), time.time() - t))
     return mpo, isite, st, L, R, prog
 
 
 def time_stages(prog, mpo, L, R, psi, sig, dtype, reps=3):
-    """Per-stage average launch time with HIP events on the launch stream."""
+    """Per-launch average time of each sigma kernel, with HIP events recorded
+    on the stream the kernels are launched on, plus its algorithmic FLOPs and
+    bytes.  Launches (per op chunk): L*psi, then either the chained
+    W[i]*W[i+1] launch or its two separate stages, then *R into sigma."""
     from zmpo_dmrg_amd import device as dv
-    from zmpo_dmrg_amd.qtensor import run_program
+    from zmpo_dmrg_amd.qtensor import run_chain, run_program
     torch = dv.torch()
-    t1, t2, t3 = prog.buffers(dtype)
+    fused = prog.fused
+    t1, t2, t3 = prog.buffers(dtype, mid=not fused)
     Lb, Rb = L.buffer(dtype), R.buffer(dtype)
     Wi, Wj = mpo.buffer(prog.isite, dtype), mpo.buffer(prog.isite + 1, dtype)
-    ops = [(0, Lb, psi, t1), (1, t1, Wi, t2), (2, t2, Wj, t3), (3, t3, Rb, sig)]
-    ms = np.zeros(4)
-    flops = np.zeros(4)
-    n = 0
+    elsize = 8 if dtype == 0 else 4
+    if fused:
+        names = ['L*psi', 'W[i]*W[i+1] chained', '*R']
+    else:
+        names = ['L*psi', '*W[i]', '*W[i+1]', '*R']
+    n = len(names)
+    ms, flops, nbytes = np.zeros(n), np.zeros(n), np.zeros(n)
+    stream = dv.stream_obj()
     for _ in range(reps):
-        for st in prog.stages:
-            for k, A, B, C in ops:
+        for st, ch in zip(prog.stages, prog.chains):
+            if fused:
+                ops = [(lambda: run_program(st[0], Lb, psi, t1, dtype, beta=0), st[0]),
+                       (lambda: run_chain(ch, t1, Wi, Wj, t3, dtype), ch),
+                       (lambda: run_program(st[3], t3, Rb, sig, dtype, beta=0), st[3])]
+            else:
+                ops = [(lambda: run_program(st[0], Lb, psi, t1, dtype, beta=0), st[0]),
+                       (lambda: run_program(st[1], t1, Wi, t2, dtype, beta=0), st[1]),
+                       (lambda: run_program(st[2], t2, Wj, t3, dtype, beta=0), st[2]),
+                       (lambda: run_program(st[3], t3, Rb, sig, dtype, beta=0), st[3])]
+            for k, (fn, pg) in enumerate(ops):
                 ev0 = torch.cuda.Event(enable_timing=True)
                 ev1 = torch.cuda.Event(enable_timing=True)
-                ev0.record()
-                run_program(st[k], A, B, C, dtype, beta=0)
-                ev1.record()
+                ev0.record(stream)
+                fn()
+                ev1.record(stream)
                 ev1.synchronize()
                 ms[k] += ev0.elapsed_time(ev1)
-                flops[k] += st[k].flops
-        n += 1
-    return ms / n, flops / n
+                flops[k] += pg.flops
+                nbytes[k] += pg.alg_bytes(elsize)
+    return names, ms / reps, flops / reps, nbytes / reps
+
+
+def traffic_per_launch(args, launch):
+    """Measured HBM bytes per launch of the dominant stage kernel, from the
+    calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE summary kept in profiles/
+    for this workload (None when no such measurement exists)."""
+    path = os.path.join(ROOT, 'profiles', 'traffic_sigma_k%d_d%d_%s.json' % (args.K, args.D, args.dtype))
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = d.get('launches', {}).get(launch)
+    if not rec:
+        return None
+    return rec.get('bytes_per_launch')
 
 
 def cpu_baseline(mpo, isite, st, L, R, nops_sample, psi_host):
@@ -156,10 +190,12 @@ def main():
     ap.add_argument('--D', type=int, default=1000)
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--dtype', choices=['f64', 'f32'], default='f64')
-    ap.add_argument('--budget-gb', type=float, default=48.0)
+    ap.add_argument('--budget-gb', type=float, default=96.0)
     ap.add_argument('--cpu-ops', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--kernel', choices=['shaped', 'wave', 'tile'], default=None,
+    ap.add_argument('--unfused', action='store_true',
+                    help='run sigma stages 2 and 3 as two launches through HBM')
+    ap.add_argument('--kernel', choices=['stream', 'shaped', 'wave', 'tile'], default=None,
                     help='grouped-GEMM schedule (default: library default)')
     args = ap.parse_args()
     if args.gpus != 1:
@@ -174,7 +210,8 @@ def main():
     torch = dv.torch()
     dtype = 0 if args.dtype == 'f64' else 1
     mpo, isite, st, L, R, prog = build_workload(args.K, args.D, args.seed, dtype,
-                                                int(args.budget_gb * 2 ** 30))
+                                                int(args.budget_gb * 2 ** 30),
+                                                fused=not args.unfused)
     rng = np.random.default_rng(1)
     psi_h = rng.standard_normal(st.qtmp.size_allowed)
     psi_h /= np.linalg.norm(psi_h)
@@ -192,16 +229,32 @@ def main():
     ms_step = 1e3 * dt / args.steps
     gflops = prog.flops * args.steps / dt / 1e9
     log('timed %d sigma steps: %.3f ms/step, %.1f GFLOP/s' % (args.steps, ms_step, gflops))
-    stage_ms, stage_fl = time_stages(prog, mpo, L, R, psi, sig, dtype)
+    names, stage_ms, stage_fl, stage_by = time_stages(prog, mpo, L, R, psi, sig, dtype)
     dom = int(np.argmax(stage_ms))
     nlaunch = len(prog.stages)
     avg_ms = stage_ms[dom] / nlaunch
-    ach = stage_fl[dom] / nlaunch / (avg_ms * 1e-3)
-    roof = {'kernel': '%s grouped GEMM (sigma stage %d of 4)' % (pq.KERNEL, dom + 1), 'bound': 'mfma',
-            'achieved': ach / 1e12, 'peak': PEAK[dtype] / 1e12, 'unit': 'TFLOP/s',
-            'frac': ach / PEAK[dtype], 'traffic': None,
-            'launch_ms': avg_ms, 'stage_ms': [round(x, 4) for x in stage_ms],
-            'stage_gflop': [round(x / 1e9, 3) for x in stage_fl]}
+    ach_b = stage_by[dom] / nlaunch / (avg_ms * 1e-3)
+    ach_f = stage_fl[dom] / nlaunch / (avg_ms * 1e-3)
+    # the stage GEMMs have K ~ 18-60 per block pair: arithmetic intensity
+    # (FLOP per algorithmic byte) is below the FP64 ridge point, so HBM bounds
+    bound = 'hbm' if stage_fl[dom] / stage_by[dom] < PEAK[dtype] / HBM else 'mfma'
+    kname = 'chained grouped GEMM (zmpo_chaingemm)' if (prog.fused and dom == 1) else \
+        '%s grouped GEMM' % pq.KERNEL
+    roof = {'kernel': '%s, sigma launch %s' % (kname, names[dom]),
+            'bound': bound,
+            'achieved': ach_b / 1e9 if bound == 'hbm' else ach_f / 1e12,
+            'peak': HBM / 1e9 if bound == 'hbm' else PEAK[dtype] / 1e12,
+            'unit': 'GB/s' if bound == 'hbm' else 'TFLOP/s',
+            'frac': ach_b / HBM if bound == 'hbm' else ach_f / PEAK[dtype],
+            'traffic': traffic_per_launch(args, names[dom]),
+            'alg_bytes_per_launch': stage_by[dom] / nlaunch,
+            'flops_per_launch': stage_fl[dom] / nlaunch,
+            'achieved_tflops': ach_f / 1e12, 'mfma_frac': ach_f / PEAK[dtype],
+            'launch_ms': avg_ms, 'launches_per_step': nlaunch,
+            'stage_names': names,
+            'stage_ms': [round(x, 4) for x in stage_ms],
+            'stage_gflop': [round(x / 1e9, 3) for x in stage_fl],
+            'stage_alg_gb': [round(x / 1e9, 3) for x in stage_by]}
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -223,7 +276,8 @@ def main():
         'data': 'synthetic (seeded integrals; synthetic block-sparse MPS, device-built envs)',
         'config': {'workload': 'middle-bond two-site sigma, K=%d N=%d Sz=0 D=%d, nops=%d' % (
             args.K, args.K, args.D, mpo.nops), 'ndim0': int(st.qtmp.size_allowed),
-            'sigma_gflop': prog.flops / 1e9, 'chunks': len(prog.chunks)},
+            'sigma_gflop': prog.flops / 1e9, 'chunks': len(prog.chunks),
+            'launches': ' | '.join(names)},
         'roofline': roof, 'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)

@@ -134,6 +134,59 @@ int zmpo_shapedgemm(int dtype, int shape, const zmpo_task* tasks, const zmpo_uni
                     const zmpo_pair* pairs, const void* A, const void* B, void* C, void* work,
                     int beta, void* stream);
 
+/* One output tile of the streaming schedule, with the geometry of its task
+ * folded in (no dependent task lookup on the device).  Rows r < M and
+ * columns c < N of the block are real; the tile covers rows [m0, m0+TM) and
+ * columns [n0, n0+TN).  slot >= 0 sends the tile to workspace slot `slot`
+ * (split K) instead of C. */
+typedef struct zmpo_sunit {
+  int64_t c_off, c_smhi;
+  int32_t c_mlo, c_smlo, c_sn;
+  int32_t m0, n0, M, N;
+  int32_t p_begin, p_end, slot;
+  int32_t pad0, pad1;
+} zmpo_sunit;
+
+/* Streaming grouped GEMM: workgroup w runs units [w*upw, (w+1)*upw) (upw in
+ * [1, 64]) as one software pipeline over all their (pair, k-tile) items --
+ * the operand tiles of the next item are in flight while the current one
+ * is multiplied and while finished tiles are written -- with k-tiles of kt
+ * (16 or 32) and the tile shapes of zmpo_shapedgemm.  Split-K slots are
+ * summed afterwards by the zmpo_shapedgemm reduction (groups over tasks).
+ * Same pair semantics and results as zmpo_shapedgemm. */
+int zmpo_streamgemm(int dtype, int shape, int kt, const zmpo_sunit* units, int64_t nunits,
+                    int upw, const zmpo_task* tasks, const zmpo_rgroup* groups,
+                    int64_t ngroups, const zmpo_pair* pairs, const void* A, const void* B,
+                    void* C, void* work, int beta, void* stream);
+
+/* Two chained grouped GEMMs over row groups, C = (A B1) B2 blockwise, for
+ * the middle sigma stages of HVecQt (mpo_dmrg_kernelQt.py:277-296: top1 x
+ * W[i] -> top2, top2 x W[i+1] -> top3).  All blocks of one row group share
+ * their M rows (the bra and right-bond sectors), so for a tile of 16*rb rows
+ * the group's whole intermediate (every first-stage output block, `ncols`
+ * columns) stays in LDS and feeds the second stage; it never touches HBM.
+ *   first-stage tasks  [t1_begin, t1_end): c_off = intermediate column
+ *     offset, N = its width; pairs: a_off/A maps relative to a_base, b_off
+ *     relative to b1_base.
+ *   second-stage tasks [t2_begin, t2_end): c_off and C maps relative to
+ *     c_base; pairs: a_off = intermediate column offset (A maps unused),
+ *     b_off relative to b2_base.
+ * One workgroup per zmpo_cunit (rows [m0, m0 + 16*rb) of group `group`). */
+typedef struct zmpo_cgroup {
+  int32_t M, ncols, t1_begin, t1_end, t2_begin, t2_end;
+} zmpo_cgroup;
+
+typedef struct zmpo_cunit {
+  int64_t a_base, b1_base, b2_base, c_base;
+  int32_t group, m0;
+} zmpo_cunit;
+
+/* rb in {1, 2}; ncols_max >= every group's ncols (sizes the LDS). */
+int zmpo_chaingemm(int dtype, int rb, int ncols_max, const zmpo_cunit* units, int64_t nunits,
+                   const zmpo_cgroup* groups, const zmpo_task* tasks, const zmpo_pair* pairs,
+                   const void* A, const void* B1, const void* B2, void* C, int beta,
+                   void* stream);
+
 /* out[j] = sum_i X[j*ldx + i] * y[i] for j < nx (device out, float64;
  * work holds zmpo_dots_workspace(nx) doubles).  Fixed reduction order. */
 int zmpo_dots(int dtype, const void* X, int64_t ldx, int nx, const void* y, int64_t n,

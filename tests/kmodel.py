@@ -90,6 +90,149 @@ def emulate_wave(prog, A, B, C, dtype=0, beta=0, stream=None, kernel=None):
         C[ci] = v if beta == 0 else C[ci] + v
 
 
+def emulate_stream(prog, A, B, C, kt, upw, beta=0):
+    """Model of zmpo_streamgemm: the per-workgroup cursor over (unit, pair,
+    k-tile) items of the stream kernel, with its fetch/compute/epilogue order,
+    on the zmpo_sunit records; split tiles summed per group in slot order."""
+    from zmpo_dmrg_amd.qtensor import SHAPES, make_schedule, stream_units
+    A, B, C = _np(A), _np(B), _np(C)
+    TMs, TNs = SHAPES[prog.shape]
+    units, groups, nslots = make_schedule(prog.tasks_host, prog.pairs_host, prog.rb,
+                                          reuse_key=getattr(prog, 'reuse_key', None),
+                                          tile=(TMs, TNs))
+    su = stream_units(prog.tasks_host, units)
+    p = prog.pairs_host
+    work = np.zeros((max(nslots, 1), TMs, TNs))
+    nitems = 0
+    for u0 in range(0, len(su), upw):
+        rng_ = su[u0:u0 + upw]
+        n = len(rng_)
+        st = {}
+
+        def setup_pair():
+            q = p[st['lp']]
+            un = rng_[st['lu']]
+            m = un['m0'] + np.arange(TMs)
+            nn = un['n0'] + np.arange(TNs)
+            mv, nv = m < un['M'], nn < un['N']
+            st['ai'] = np.where(mv, q['a_off'] + (m // q['a_mlo']) * q['a_smhi']
+                                + (m % q['a_mlo']) * q['a_smlo'], -1)
+            st['bi'] = np.where(nv, q['b_off'] + (nn // q['b_nlo']) * q['b_snhi']
+                                + (nn % q['b_nlo']) * q['b_snlo'], -1)
+            st['q'] = q
+            st['lK'] = int(q['K'])
+
+        def begin_unit():
+            un = rng_[st['lu']]
+            st['lp'], st['lpe'], st['lkt'], st['lK'] = int(un['p_begin']), int(un['p_end']), 0, 0
+            if st['lp'] < st['lpe']:
+                setup_pair()
+
+        def fetch():
+            unit = st['lu']
+            lK, lkt = st['lK'], st['lkt']
+            if lK > 0:
+                q = st['q']
+                k = lkt * kt + np.arange(kt)
+                kv = k < lK
+                ai, bi = st['ai'], st['bi']
+                Am = np.where((ai[:, None] >= 0) & kv[None, :],
+                              A[np.maximum(ai[:, None] + k[None, :] * q['a_sk'], 0)], 0.0) * q['scale']
+                Bm = np.where(kv[:, None] & (bi[None, :] >= 0),
+                              B[np.maximum(k[:, None] * q['b_sk'] + bi[None, :], 0)], 0.0)
+                tile = (Am, Bm)
+            else:
+                tile = None
+            more_k = lK > 0 and (lkt + 1) * kt < lK
+            more_p = lK > 0 and st['lp'] + 1 < st['lpe']
+            last = not more_k and not more_p
+            if more_k:
+                st['lkt'] += 1
+            elif more_p:
+                st['lp'] += 1
+                st['lkt'] = 0
+                setup_pair()
+            else:
+                st['lu'] += 1
+                if st['lu'] < n:
+                    begin_unit()
+            return tile, last, unit
+
+        st['lu'] = 0
+        begin_unit()
+        cur = fetch()
+        acc = np.zeros((TMs, TNs))
+        while True:
+            nvalid = st['lu'] < n
+            nxt = fetch() if nvalid else None
+            tile, last, unit = cur
+            nitems += 1
+            if tile is not None:
+                acc += tile[0] @ tile[1]
+            if last:
+                un = rng_[unit]
+                if un['slot'] >= 0:
+                    work[un['slot']] = acc
+                else:
+                    m = un['m0'] + np.arange(TMs)
+                    nn = un['n0'] + np.arange(TNs)
+                    mv, nv = m < un['M'], nn < un['N']
+                    ci = (un['c_off'] + (m[mv] // un['c_mlo']) * un['c_smhi']
+                          + (m[mv] % un['c_mlo']) * un['c_smlo'])[:, None] + nn[nv][None, :] * un['c_sn']
+                    v = acc[np.ix_(mv, nv)]
+                    C[ci] = v if beta == 0 else C[ci] + v
+                acc = np.zeros((TMs, TNs))
+            if not nvalid:
+                break
+            cur = nxt
+    t = prog.tasks_host
+    for g in groups:
+        tk = t[g['task']]
+        s_ = work[g['slot_begin']:g['slot_begin'] + g['nslots']].sum(0)
+        m = g['m0'] + np.arange(TMs)
+        nn = g['n0'] + np.arange(TNs)
+        mv, nv = m < tk['M'], nn < tk['N']
+        ci = (tk['c_off'] + (m[mv] // tk['c_mlo']) * tk['c_smhi']
+              + (m[mv] % tk['c_mlo']) * tk['c_smlo'])[:, None] + nn[nv][None, :] * tk['c_sn']
+        v = s_[np.ix_(mv, nv)]
+        C[ci] = v if beta == 0 else C[ci] + v
+    return nitems
+
+
+def emulate_chain(prog, A, B1, B2, C, beta=0):
+    """Model of zmpo_chaingemm: per unit, the group's intermediate columns
+    from the first-stage tasks, then the output blocks from it."""
+    A, B1, B2, C = _np(A), _np(B1), _np(B2), _np(C)
+    t, p, g = prog.tasks_host, prog.pairs_host, prog.groups_host
+    tm = 16 * prog.rb
+    for u in prog.units_host:
+        gg = g[u['group']]
+        mv = min(tm, int(gg['M']) - int(u['m0']))
+        m = int(u['m0']) + np.arange(mv)
+        mid = np.zeros((mv, max(int(gg['ncols']), 1)))
+        for ti in range(int(gg['t1_begin']), int(gg['t1_end'])):
+            tk = t[ti]
+            n = np.arange(int(tk['N']))
+            acc = np.zeros((mv, len(n)))
+            for q in p[int(tk['p_begin']):int(tk['p_end'])]:
+                k = np.arange(int(q['K']))
+                ai = u['a_base'] + q['a_off'] + (m // q['a_mlo']) * q['a_smhi'] + (m % q['a_mlo']) * q['a_smlo']
+                bi = u['b1_base'] + q['b_off'] + (n // q['b_nlo']) * q['b_snhi'] + (n % q['b_nlo']) * q['b_snlo']
+                acc += q['scale'] * (A[ai[:, None] + k[None, :] * q['a_sk']] @ B1[k[:, None] * q['b_sk'] + bi[None, :]])
+            mid[:, int(tk['c_off']):int(tk['c_off']) + len(n)] = acc
+        for ti in range(int(gg['t2_begin']), int(gg['t2_end'])):
+            tk = t[ti]
+            n = np.arange(int(tk['N']))
+            acc = np.zeros((mv, len(n)))
+            for q in p[int(tk['p_begin']):int(tk['p_end'])]:
+                k = np.arange(int(q['K']))
+                bi = u['b2_base'] + q['b_off'] + (n // q['b_nlo']) * q['b_snhi'] + (n % q['b_nlo']) * q['b_snlo']
+                acc += q['scale'] * (mid[:, int(q['a_off']) + k] @ B2[k[:, None] * q['b_sk'] + bi[None, :]])
+            ci = (u['c_base'] + tk['c_off'] + (m // tk['c_mlo']) * tk['c_smhi']
+                  + (m % tk['c_mlo']) * tk['c_smlo'])[:, None] + n[None, :] * tk['c_sn']
+            C[ci] = acc if beta == 0 else C[ci] + acc
+
+
 def _gather(dst, src, idx, *a):
     d, s_, i = _np(dst), _np(src), _np(idx)
     d[:] = np.where(i >= 0, s_[np.maximum(i, 0)], 0.0)
@@ -148,6 +291,8 @@ def host_model():
     patch(pd, '_VecOps', _CpuVecOps)
     patch(qq, 'run_program', emulate_wave)
     patch(kq, 'run_program', emulate_wave)
+    patch(kq, 'run_chain', lambda prog, A, B1, B2, C, dtype=0, beta=0, stream=None:
+          emulate_chain(prog, A, B1, B2, C, beta))
     patch(qq, 'gather', _gather)
     patch(kq, 'gather', _gather)
     try:

@@ -24,7 +24,7 @@ def _dense_program(M, N, K, trans_a=False, trans_b=False):
 @pytest.mark.parametrize('shape', [(64, 64, 16), (37, 90, 33), (130, 7, 200), (1, 1, 1),
                                    (5, 9, 5000)])
 @pytest.mark.parametrize('ta,tb', [(False, False), (True, False), (False, True)])
-@pytest.mark.parametrize('kernel', ['wave', 'tile', 'shaped'])
+@pytest.mark.parametrize('kernel', ['wave', 'tile', 'shaped', 'stream'])
 def test_blockgemm_dense_f64(gpu, shape, ta, tb, kernel):
     M, N, K = shape
     rng = np.random.default_rng(M * N + K)
@@ -63,29 +63,37 @@ def test_wavegemm_split_k_random_programs(gpu):
         npair = int(rng.integers(0, 6))
         tasks[i] = (off, 0, M, N, M, 0, N, 1, len(pairs), len(pairs) + npair, 0, 0)
         off += M * N
-        for _ in range(npair):
+        for j in range(npair):
             K = int(rng.integers(1, 3000))
+            # consecutive pairs alternate between K-contiguous and
+            # row-contiguous operands (A and B independently)
+            a = (M, 0, K, 1) if (i + j) % 2 else (M, 0, 1, M)
+            b = (N, 0, 1, N) if (i + j) % 3 else (N, 0, K, 1)
             pairs.append((int(rng.integers(0, 1000)), int(rng.integers(0, 1000)),
-                          float(rng.standard_normal()), K, M, 0, K, 1, N, 0, 1, N, 0))
+                          float(rng.standard_normal()), K) + a + b + (0,))
     pairs = np.array(pairs, dtype=L.PAIR_DTYPE)
     A, B = rng.standard_normal(400000), rng.standard_normal(400000)
     want = np.zeros(off)
     prog = pq.Program(tasks, pairs)
     emulate(prog, A, B, want)
-    for kernel, rb, shape in [('wave', r, 0) for r in (1, 2, 4)] + [('shaped', 1, s) for s in pq.SHAPES]:
+    configs = [('wave', r, 0, 0, 0) for r in (1, 2, 4)] + [('shaped', 1, s, 0, 0) for s in pq.SHAPES]
+    configs += [('stream', 1, s, kt, upw) for s in pq.SHAPES for kt in (16, 32) for upw in (1, 3, 64)]
+    for kernel, rb, shape, kt, upw in configs:
         prog = pq.Program(tasks, pairs)
-        prog.rb, prog.shape = rb, shape
+        prog.rb, prog.shape, prog.kt, prog.upw = rb, shape, kt or None, upw or None
         C = dv.zeros(off, 0)
         pq.run_program(prog, dv.upload(A, 0), dv.upload(B, 0), C, 0, kernel=kernel)
-        assert np.abs(dv.download(C) - want).max() < 1e-10, (kernel, rb, shape)
+        assert np.abs(dv.download(C) - want).max() < 1e-10, (kernel, rb, shape, kt, upw)
 
 
-def test_blockgemm_beta_accumulates(gpu):
+@pytest.mark.parametrize('kernel', ['wave', 'tile', 'shaped', 'stream'])
+def test_blockgemm_beta_accumulates(gpu, kernel):
     M = N = K = 20
     rng = np.random.default_rng(2)
     A, B, C0 = (rng.standard_normal(n) for n in (M * K, K * N, M * N))
     C = dv.upload(C0, 0)
-    pq.run_program(_dense_program(M, N, K), dv.upload(A, 0), dv.upload(B, 0), C, 0, beta=1)
+    pq.run_program(_dense_program(M, N, K), dv.upload(A, 0), dv.upload(B, 0), C, 0, beta=1,
+                   kernel=kernel)
     ref = C0.reshape(M, N) + A.reshape(M, K) @ B.reshape(K, N)
     assert np.abs(dv.download(C).reshape(M, N) - ref).max() < 1e-12
 
@@ -116,9 +124,13 @@ def test_blockgemm_real_plans_match_model(gpu, case4):
             A, B = rng.standard_normal(na), rng.standard_normal(nb)
             want = np.zeros(nc)
             emulate(pg, A, B, want)
-            C = dv.zeros(nc, 0)
-            pq.run_program(pg, dv.upload(A, 0), dv.upload(B, 0), C, 0)
-            assert np.abs(dv.download(C) - want).max() < 1e-12 * max(1.0, np.abs(want).max())
+            for kernel, upw in (('shaped', None), ('stream', None), ('stream', 64)):
+                pg.upw = upw
+                pg._sched = None
+                C = dv.zeros(nc, 0)
+                pq.run_program(pg, dv.upload(A, 0), dv.upload(B, 0), C, 0, kernel=kernel)
+                err = np.abs(dv.download(C) - want).max()
+                assert err < 1e-12 * max(1.0, np.abs(want).max()), (kernel, upw, k)
 
 
 def test_vector_kernels(gpu):
@@ -161,3 +173,36 @@ def test_errors_surface(gpu):
     rc = lib.zmpo_blockgemm(7, None, 1, None, 1, None, None, None, 0, None)
     assert rc != 0
     assert lib.zmpo_last_error()
+
+
+@pytest.mark.parametrize('dtype', [0, 1])
+@pytest.mark.parametrize('rb', [1, 2])
+def test_chaingemm_real_plan_matches_two_stages(gpu, rb, dtype):
+    """zmpo_chaingemm (sigma stages 2-3 with the intermediate in LDS) vs the
+    CPU model of the two separate stages, on a real K=5 sigma plan."""
+    from kmodel import host_model
+    from zmpo_dmrg_amd import kernelQt as kq
+    case5 = Case(K=5, D=24, seed=4, sweeps=1, nelec=4)
+    with host_model():
+        Lh, Rh = case5.product_envs()
+    isite = 2
+    qkey, _, pt, _ = case5.templates(isite)
+    sp = kq.SigmaProgram(case5.mpo, isite, pt, qkey, Lh[isite - 1], Rh[isite + 2],
+                         case5.mpo.hpwts, fused=True)
+    rng = np.random.default_rng(rb)
+    for st, ch in zip(sp.stages, sp.chains):
+        ch = ch.with_rb(rb)
+        n1 = int(st[1].pairs_host['a_off'].max() + 5000)
+        nw = int(max(st[1].pairs_host['b_off'].max(), st[2].pairs_host['b_off'].max()) + 5000)
+        n2 = int(st[1].tasks_host['c_off'].max() + 5000)
+        n3 = int(st[2].tasks_host['c_off'].max() + 5000)
+        A, Wi, Wj = rng.standard_normal(n1), rng.standard_normal(nw), rng.standard_normal(nw)
+        mid = np.zeros(n2)
+        emulate(st[1], A, Wi, mid)
+        want = np.zeros(n3)
+        emulate(st[2], mid, Wj, want)
+        C = dv.zeros(n3, dtype)
+        pq.run_chain(ch, dv.upload(A, dtype), dv.upload(Wi, dtype), dv.upload(Wj, dtype), C, dtype)
+        tol = 1e-12 if dtype == 0 else 2e-5
+        err = np.abs(dv.download(C).astype(np.float64) - want).max()
+        assert err < tol * max(1.0, np.abs(want).max()), (rb, dtype, err)

@@ -134,3 +134,70 @@ def test_wave_schedule_split_k_matches_direct():
             got = np.zeros(off)
             emulate_wave(prog, A, B, got, kernel=kernel)
             assert np.abs(got - want).max() < 1e-10
+
+
+def test_stream_schedule_cursor_matches_direct():
+    """The streaming kernel's item cursor (unit ranges of 1..64, k-tiles of 16
+    and 32, empty tasks, split-K slots, beta=1) reproduces the direct sum."""
+    import zmpo_dmrg_amd._lib as L
+    from kmodel import emulate_stream
+    rng = np.random.default_rng(5)
+    nt = 14
+    tasks = np.zeros(nt, L.TASK_DTYPE)
+    pairs, off = [], 0
+    for i in range(nt):
+        M, N = int(rng.integers(1, 140)), int(rng.integers(1, 70))
+        npair = int(rng.integers(0, 4)) if i % 5 else 0
+        tasks[i] = (off, 0, M, N, M, 0, N, 1, len(pairs), len(pairs) + npair, 0, 0)
+        off += M * N
+        for _ in range(npair):
+            K = int(rng.integers(1, 2600))
+            if rng.random() < 0.5:   # K-contiguous A
+                a = (M, 0, K, 1)
+            else:                    # row-contiguous A
+                a = (M, 0, 1, M)
+            pairs.append((int(rng.integers(0, 1000)), int(rng.integers(0, 1000)),
+                          float(rng.standard_normal()), K) + a + (N, 0, 1, N, 0))
+    pairs = np.array(pairs, dtype=L.PAIR_DTYPE)
+    prog = pq.Program(tasks, pairs)
+    A, B = rng.standard_normal(400000), rng.standard_normal(400000)
+    C0 = rng.standard_normal(off)
+    want = C0.copy()
+    emulate(prog, A, B, want, beta=1)
+    for shape in pq.SHAPES:
+        for kt in (16, 32):
+            for upw in (1, 3, 64):
+                prog.shape = shape
+                got = C0.copy()
+                emulate_stream(prog, A, B, got, kt, upw, beta=1)
+                assert np.abs(got - want).max() < 1e-10, (shape, kt, upw)
+
+
+@pytest.mark.parametrize('rb', [1, 2])
+def test_chain_program_matches_two_stages(rb):
+    """Sigma stages 2-3 as one chained program (row groups, intermediate
+    columns) equal the two separate grouped GEMMs on a real K=5 plan."""
+    from kmodel import emulate_chain
+    case5 = Case(K=5, D=24, seed=4, sweeps=1, nelec=4)
+    with host_model():
+        Lh, Rh = case5.product_envs()
+        isite = 2
+        qkey, _, pt, _ = case5.templates(isite)
+        sp = kq.SigmaProgram(case5.mpo, isite, pt, qkey, Lh[isite - 1], Rh[isite + 2],
+                             case5.mpo.hpwts, fused=True)
+    rng = np.random.default_rng(1)
+    for st, ch in zip(sp.stages, sp.chains):
+        ch = ch.with_rb(rb)
+        n1 = int(st[1].pairs_host['a_off'].max() + 5000)
+        nw = int(max(st[1].pairs_host['b_off'].max(), st[2].pairs_host['b_off'].max()) + 5000)
+        n2 = int(st[1].tasks_host['c_off'].max() + 5000)
+        n3 = int(st[2].tasks_host['c_off'].max() + 5000)
+        A, Wi, Wj = rng.standard_normal(n1), rng.standard_normal(nw), rng.standard_normal(nw)
+        mid = np.zeros(n2)
+        emulate(st[1], A, Wi, mid)
+        want = np.zeros(n3)
+        emulate(st[2], mid, Wj, want)
+        got = np.zeros(n3)
+        emulate_chain(ch, A, Wi, Wj, got)
+        assert np.abs(got - want).max() < 1e-11 * max(1.0, np.abs(want).max())
+        assert ch.flops == st[1].flops + st[2].flops

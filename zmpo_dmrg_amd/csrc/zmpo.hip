@@ -562,6 +562,436 @@ int dispatch_shaped(int shape, const zmpo_task* tasks, const zmpo_unit* units, i
   }
 }
 
+// ------------------------------------------------------------ streaming tiles
+// A workgroup runs a contiguous range of units as one software pipeline: the
+// operand tiles of item i+1 (an item = one KT-deep slice of one pair of one
+// unit) are loaded into registers while item i is multiplied from LDS and
+// while a finished unit is written out, so per-unit latency (descriptor,
+// operand and store round trips) overlaps across the range instead of
+// serialising every small block product.
+constexpr int MAXUPW = 64;
+
+// Register-staged operand loader.  Element e (< ROWS*KT) of the KTxROWS
+// tile belongs to thread e % 256; for a K-contiguous operand (sk == 1)
+// consecutive threads walk k, otherwise they walk rows, so that consecutive
+// threads read consecutive addresses.
+template <typename T, int ROWS, int KT>
+struct StreamLoader {
+  static constexpr int NE = (ROWS * KT + 255) / 256;
+  int roff[NE];  // row offset relative to the block, -1 = padding row
+  T v[NE];
+  const T* base;
+  int64_t ks;
+  int K;
+  bool kmaj;   // element layout of the pair set up last
+  bool vkmaj;  // element layout of the values held in v (set by fetch)
+  T scale;
+  __device__ __forceinline__ static int kidx(bool km, int e) { return km ? e % KT : e / ROWS; }
+  __device__ __forceinline__ static int ridx(bool km, int e) { return km ? e / KT : e % ROWS; }
+  __device__ __forceinline__ void setup(int tid, const T* b, int64_t off, int K_, int64_t sk,
+                                        int mlo, int64_t shi, int64_t slo, int r0, int rmax,
+                                        T sc) {
+    base = b + off;
+    ks = sk;
+    K = K_;
+    kmaj = sk == 1;
+    scale = sc;
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      const int e = tid + 256 * i;
+      const int r = r0 + ridx(kmaj, e);
+      roff[i] = (e < ROWS * KT && r < rmax) ? (int)((r / mlo) * shi + (r % mlo) * slo) : -1;
+    }
+  }
+  // The cursor may set up the next pair before the fetched values are
+  // stored, so store() uses the layout recorded here.
+  __device__ __forceinline__ void fetch(int tid, int k0) {
+    vkmaj = kmaj;
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      const int e = tid + 256 * i;
+      const int k = k0 + kidx(kmaj, e);
+      v[i] = (roff[i] >= 0 && k < K) ? base[roff[i] + (int64_t)k * ks] * scale : (T)0;
+    }
+  }
+  __device__ __forceinline__ void store(int tid, T (*dst)[ROWS + 1]) const {
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      const int e = tid + 256 * i;
+      if (e < ROWS * KT) dst[kidx(vkmaj, e)][ridx(vkmaj, e)] = v[i];
+    }
+  }
+};
+
+template <typename T, int TMv, int TNv, int KT>
+__global__ __launch_bounds__(256) void stream_gemm_kernel(
+    const zmpo_sunit* __restrict__ units, int64_t nunits, int upw,
+    const zmpo_pair* __restrict__ pairs, const T* __restrict__ A, const T* __restrict__ B,
+    T* __restrict__ C, T* __restrict__ work, int beta) {
+  typedef typename Acc<T>::type accv;
+  constexpr int BM = TMv / 16, BN = TNv / 16;
+  constexpr int NBLK = BM * BN;
+  constexpr int PER = (NBLK + 3) / 4;
+  constexpr int ASZ = KT * (TMv + 1), STG = TMv * TNv;
+  constexpr int ABUF = ASZ > STG ? ASZ : STG;
+  // the A tile buffer doubles as the output stage of a finished unit
+  __shared__ T Abuf[ABUF];
+  __shared__ T Bs[KT][TNv + 1];
+  __shared__ zmpo_sunit su[MAXUPW];
+  T(*As)[TMv + 1] = reinterpret_cast<T(*)[TMv + 1]>(Abuf);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t u0 = xcd_logical_block(blockIdx.x, gridDim.x) * (int64_t)upw;
+  if (u0 >= nunits) return;  // workgroup-uniform
+  const int n = (int)min((int64_t)upw, nunits - u0);
+  {
+    static_assert(sizeof(zmpo_sunit) == 64, "zmpo_sunit layout");
+    const uint4* src = reinterpret_cast<const uint4*>(units + u0);
+    uint4* dst = reinterpret_cast<uint4*>(su);
+    for (int e = tid; e < 4 * n; e += 256) dst[e] = src[e];
+  }
+  __syncthreads();
+
+  StreamLoader<T, TMv, KT> la;
+  StreamLoader<T, TNv, KT> lb;
+#pragma unroll
+  for (int i = 0; i < la.NE; ++i) la.v[i] = (T)0;
+#pragma unroll
+  for (int i = 0; i < lb.NE; ++i) lb.v[i] = (T)0;
+  la.vkmaj = lb.vkmaj = true;
+  accv acc[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) acc[i] = accv{0, 0, 0, 0};
+
+  // load cursor: unit lu of the range, pair lp (< lpe), k-tile lkt; lK = 0
+  // marks a unit without pairs (one empty item that writes zeros)
+  int lu = 0, lp = 0, lpe = 0, lkt = 0, lK = 0;
+#define ZMPO_STREAM_PAIR()                                                                    \
+  {                                                                                           \
+    const zmpo_pair pr = pairs[lp];                                                           \
+    la.setup(tid, A, pr.a_off, pr.K, pr.a_sk, pr.a_mlo, pr.a_smhi, pr.a_smlo, su[lu].m0,      \
+             su[lu].M, (T)pr.scale);                                                          \
+    lb.setup(tid, B, pr.b_off, pr.K, pr.b_sk, pr.b_nlo, pr.b_snhi, pr.b_snlo, su[lu].n0,      \
+             su[lu].N, (T)1);                                                                 \
+    lK = pr.K;                                                                                \
+  }
+#define ZMPO_STREAM_UNIT()          \
+  {                                 \
+    lp = su[lu].p_begin;            \
+    lpe = su[lu].p_end;             \
+    lkt = 0;                        \
+    lK = 0;                         \
+    if (lp < lpe) ZMPO_STREAM_PAIR(); \
+  }
+  // fetch the item under the cursor into registers, report its depth and
+  // whether it closes its unit, then advance the cursor
+#define ZMPO_STREAM_FETCH(KS, LAST, UNIT)                                        \
+  {                                                                              \
+    UNIT = lu;                                                                   \
+    if (lK > 0) {                                                                \
+      la.fetch(tid, lkt * KT);                                                   \
+      lb.fetch(tid, lkt * KT);                                                   \
+      KS = min(KT, lK - lkt * KT);                                               \
+    } else {                                                                     \
+      KS = 0;                                                                    \
+    }                                                                            \
+    const bool more_k = lK > 0 && (lkt + 1) * KT < lK;                           \
+    const bool more_p = lK > 0 && lp + 1 < lpe;                                  \
+    LAST = !more_k && !more_p;                                                   \
+    if (more_k) {                                                                \
+      ++lkt;                                                                     \
+    } else if (more_p) {                                                         \
+      ++lp;                                                                      \
+      lkt = 0;                                                                   \
+      ZMPO_STREAM_PAIR();                                                        \
+    } else {                                                                     \
+      ++lu;                                                                      \
+      if (lu < n) ZMPO_STREAM_UNIT();                                            \
+    }                                                                            \
+  }
+
+  ZMPO_STREAM_UNIT();
+  int cks, cu;
+  bool clast;
+  ZMPO_STREAM_FETCH(cks, clast, cu);
+  la.store(tid, As);
+  lb.store(tid, Bs);
+  __syncthreads();
+  while (true) {
+    const bool nvalid = lu < n;  // uniform
+    int nks = 0, nu = 0;
+    bool nlast = false;
+    if (nvalid) ZMPO_STREAM_FETCH(nks, nlast, nu);
+#pragma unroll
+    for (int kk = 0; kk < KT; kk += 4) {
+      if (kk >= cks) break;  // uniform
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int blkid = wv + 4 * i;
+        if (blkid < NBLK) {
+          const int bm = blkid % BM, bn = blkid / BM;
+          acc[i] = Acc<T>::mfma(As[kk + lk][bm * 16 + lr], Bs[kk + lk][bn * 16 + lr], acc[i]);
+        }
+      }
+    }
+    if (clast) {
+      __syncthreads();  // every wave is done reading the A tile
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int blkid = wv + 4 * i;
+        if (blkid < NBLK) {
+          const int bm = blkid % BM, bn = blkid / BM;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            Abuf[(bm * 16 + Acc<T>::row(lk, r)) * TNv + bn * 16 + lr] = acc[i][r];
+          acc[i] = accv{0, 0, 0, 0};
+        }
+      }
+      __syncthreads();
+      const zmpo_sunit& un = su[cu];
+      if (un.slot >= 0) {
+        T* __restrict__ w = work + (int64_t)un.slot * (TMv * TNv);
+        for (int e = tid; e < TMv * TNv; e += 256) w[e] = Abuf[e];
+      } else {
+        const int mv = min(TMv, un.M - un.m0);
+        const int nv = min(TNv, un.N - un.n0);
+        for (int e = tid; e < TMv * TNv; e += 256) {
+          const int row = e / TNv, col = e % TNv;
+          if (row < mv && col < nv) {
+            const int mi = un.m0 + row, ni = un.n0 + col;
+            const int64_t off = un.c_off + (int64_t)(mi / un.c_mlo) * un.c_smhi +
+                                (int64_t)(mi % un.c_mlo) * un.c_smlo + (int64_t)ni * un.c_sn;
+            const T v = Abuf[e];
+            C[off] = beta ? C[off] + v : v;
+          }
+        }
+      }
+    }
+    if (!nvalid) break;
+    __syncthreads();  // LDS tiles (and the stage) are free
+    la.store(tid, As);
+    lb.store(tid, Bs);
+    __syncthreads();
+    cks = nks;
+    clast = nlast;
+    cu = nu;
+  }
+#undef ZMPO_STREAM_FETCH
+#undef ZMPO_STREAM_UNIT
+#undef ZMPO_STREAM_PAIR
+}
+
+template <typename T, int TMv, int TNv, int KT>
+int launch_stream(const zmpo_sunit* units, int64_t nunits, int upw, const zmpo_task* tasks,
+                  const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
+                  const void* A, const void* B, void* C, void* work, int beta, hipStream_t s) {
+  if (nunits > 0) {
+    const int64_t nwg = (nunits + upw - 1) / upw;
+    hipLaunchKernelGGL((stream_gemm_kernel<T, TMv, TNv, KT>), dim3((unsigned)nwg), dim3(256), 0,
+                       s, units, nunits, upw, pairs, (const T*)A, (const T*)B, (T*)C, (T*)work,
+                       beta);
+    int rc = check_hip(hipGetLastError(), "stream_gemm launch");
+    if (rc) return rc;
+  }
+  if (ngroups > 0) {
+    hipLaunchKernelGGL((shaped_reduce_kernel<T, TMv, TNv>), dim3((unsigned)ngroups), dim3(256), 0,
+                       s, tasks, groups, ngroups, (const T*)work, (T*)C, beta);
+    return check_hip(hipGetLastError(), "stream_reduce launch");
+  }
+  return ZMPO_OK;
+}
+
+template <typename T, int KT>
+int dispatch_stream_kt(int shape, const zmpo_sunit* units, int64_t nunits, int upw,
+                       const zmpo_task* tasks, const zmpo_rgroup* groups, int64_t ngroups,
+                       const zmpo_pair* pairs, const void* A, const void* B, void* C, void* work,
+                       int beta, hipStream_t s) {
+  switch (shape) {
+    case ZMPO_TILE_64x16: return launch_stream<T, 64, 16, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_128x16: return launch_stream<T, 128, 16, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_32x32: return launch_stream<T, 32, 32, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_64x32: return launch_stream<T, 64, 32, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_16x16: return launch_stream<T, 16, 16, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_64x64: return launch_stream<T, 64, 64, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    default: return set_error(ZMPO_ERR_ARG, "streamgemm: unknown tile shape");
+  }
+}
+
+template <typename T>
+int dispatch_stream(int shape, int kt, const zmpo_sunit* units, int64_t nunits, int upw,
+                    const zmpo_task* tasks, const zmpo_rgroup* groups, int64_t ngroups,
+                    const zmpo_pair* pairs, const void* A, const void* B, void* C, void* work,
+                    int beta, hipStream_t s) {
+  if (kt == 16)
+    return dispatch_stream_kt<T, 16>(shape, units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+  if (kt == 32)
+    return dispatch_stream_kt<T, 32>(shape, units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+  return set_error(ZMPO_ERR_ARG, "streamgemm: kt must be 16 or 32");
+}
+
+// ------------------------------------------------------------- chained GEMMs
+// One workgroup = TM (= 16*RB) rows of one row group.  Stage 1: each wave
+// takes first-stage tasks round-robin, accumulates its 16-column MFMA tiles
+// over the task's pairs (A rows from HBM, 16 consecutive rows per 128-byte
+// segment; B from the small MPO blocks) and writes them to the LDS
+// intermediate mid[col][row].  Stage 2 reads its A operand from mid and
+// writes the output blocks.  Rows past the group's M are loaded as zeros.
+extern __shared__ __align__(16) unsigned char chain_lds[];
+
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void chain_gemm_kernel(
+    const zmpo_cunit* __restrict__ units, int64_t nunits, const zmpo_cgroup* __restrict__ groups,
+    const zmpo_task* __restrict__ tasks, const zmpo_pair* __restrict__ pairs,
+    const T* __restrict__ A, const T* __restrict__ B1, const T* __restrict__ B2,
+    T* __restrict__ C, int beta) {
+  typedef typename Acc<T>::type accv;
+  constexpr int TMv = 16 * RB, LD = TMv + 1;
+  T* __restrict__ mid = reinterpret_cast<T*>(chain_lds);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t u = xcd_logical_block(blockIdx.x, gridDim.x);
+  if (u >= nunits) return;  // workgroup-uniform
+  const zmpo_cunit un = units[u];
+  const zmpo_cgroup g = groups[un.group];
+  const int mv = min(TMv, g.M - un.m0);
+
+  // ---- stage 1: intermediate tile into LDS
+  for (int t = g.t1_begin + wv; t < g.t1_end; t += 4) {
+    const zmpo_task tk = tasks[t];
+    for (int n0 = 0; n0 < tk.N; n0 += 32) {
+      accv acc[RB][2];
+#pragma unroll
+      for (int i = 0; i < RB; ++i) acc[i][0] = acc[i][1] = accv{0, 0, 0, 0};
+      for (int p = tk.p_begin; p < tk.p_end; ++p) {
+        const zmpo_pair pr = pairs[p];
+        const T* __restrict__ a = A + un.a_base + pr.a_off;
+        const T* __restrict__ b = B1 + un.b1_base + pr.b_off;
+        const T sc = (T)pr.scale;
+        int64_t aro[RB];
+        bool aok[RB];
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+          const int r = i * 16 + lr;
+          const int m = un.m0 + r;
+          aok[i] = r < mv;
+          aro[i] = aok[i] ? (int64_t)(m / pr.a_mlo) * pr.a_smhi + (int64_t)(m % pr.a_mlo) * pr.a_smlo : 0;
+        }
+        int64_t bco[2];
+        bool bok[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int n = n0 + j * 16 + lr;
+          bok[j] = n < tk.N;
+          bco[j] = bok[j] ? (int64_t)(n / pr.b_nlo) * pr.b_snhi + (int64_t)(n % pr.b_nlo) * pr.b_snlo : 0;
+        }
+#pragma unroll 4
+        for (int k0 = 0; k0 < pr.K; k0 += 4) {
+          const int k = k0 + lk;
+          const bool kok = k < pr.K;
+          T av[RB], bv[2];
+#pragma unroll
+          for (int i = 0; i < RB; ++i)
+            av[i] = (aok[i] && kok) ? a[aro[i] + (int64_t)k * pr.a_sk] * sc : (T)0;
+#pragma unroll
+          for (int j = 0; j < 2; ++j) bv[j] = (bok[j] && kok) ? b[(int64_t)k * pr.b_sk + bco[j]] : (T)0;
+#pragma unroll
+          for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = Acc<T>::mfma(av[i], bv[j], acc[i][j]);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int n = n0 + j * 16 + lr;
+        if (n < tk.N) {
+#pragma unroll
+          for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              mid[(int64_t)(tk.c_off + n) * LD + i * 16 + Acc<T>::row(lk, r)] = acc[i][j][r];
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- stage 2: output blocks from the LDS intermediate
+  for (int t = g.t2_begin + wv; t < g.t2_end; t += 4) {
+    const zmpo_task tk = tasks[t];
+    for (int n0 = 0; n0 < tk.N; n0 += 32) {
+      accv acc[RB][2];
+#pragma unroll
+      for (int i = 0; i < RB; ++i) acc[i][0] = acc[i][1] = accv{0, 0, 0, 0};
+      for (int p = tk.p_begin; p < tk.p_end; ++p) {
+        const zmpo_pair pr = pairs[p];
+        const T* __restrict__ b = B2 + un.b2_base + pr.b_off;
+        const T sc = (T)pr.scale;
+        const int c2 = (int)pr.a_off;
+        int64_t bco[2];
+        bool bok[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int n = n0 + j * 16 + lr;
+          bok[j] = n < tk.N;
+          bco[j] = bok[j] ? (int64_t)(n / pr.b_nlo) * pr.b_snhi + (int64_t)(n % pr.b_nlo) * pr.b_snlo : 0;
+        }
+#pragma unroll 4
+        for (int k0 = 0; k0 < pr.K; k0 += 4) {
+          const int k = k0 + lk;
+          const bool kok = k < pr.K;
+          T av[RB], bv[2];
+#pragma unroll
+          for (int i = 0; i < RB; ++i) av[i] = kok ? mid[(c2 + k) * LD + i * 16 + lr] * sc : (T)0;
+#pragma unroll
+          for (int j = 0; j < 2; ++j) bv[j] = (bok[j] && kok) ? b[(int64_t)k * pr.b_sk + bco[j]] : (T)0;
+#pragma unroll
+          for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = Acc<T>::mfma(av[i], bv[j], acc[i][j]);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int n = n0 + j * 16 + lr;
+        if (n < tk.N) {
+#pragma unroll
+          for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int row = i * 16 + Acc<T>::row(lk, r);
+              if (row < mv) {
+                const int m = un.m0 + row;
+                const int64_t off = un.c_base + tk.c_off + (int64_t)(m / tk.c_mlo) * tk.c_smhi +
+                                    (int64_t)(m % tk.c_mlo) * tk.c_smlo + (int64_t)n * tk.c_sn;
+                const T v = acc[i][j][r];
+                C[off] = beta ? C[off] + v : v;
+              }
+            }
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int RB>
+int launch_chain(int ncols_max, const zmpo_cunit* units, int64_t nunits, const zmpo_cgroup* groups,
+                 const zmpo_task* tasks, const zmpo_pair* pairs, const void* A, const void* B1,
+                 const void* B2, void* C, int beta, hipStream_t s) {
+  const size_t lds = (size_t)ncols_max * (16 * RB + 1) * sizeof(T);
+  if (lds > 160 * 1024) return set_error(ZMPO_ERR_ARG, "chaingemm: intermediate exceeds 160 KiB of LDS");
+  if (lds > 64 * 1024) {
+    int rc = check_hip(hipFuncSetAttribute((const void*)chain_gemm_kernel<T, RB>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                       "chaingemm: LDS attribute");
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL((chain_gemm_kernel<T, RB>), dim3((unsigned)nunits), dim3(256), lds, s, units,
+                     nunits, groups, tasks, pairs, (const T*)A, (const T*)B1, (const T*)B2, (T*)C,
+                     beta);
+  return check_hip(hipGetLastError(), "chain_gemm launch");
+}
+
 // ---------------------------------------------------------------- vectors
 // out[j] = sum_i X[j*ldx + i] * y[i], j < nx.  Deterministic two-level
 // reduction: block b sums a fixed contiguous chunk, the second pass adds the
@@ -800,6 +1230,46 @@ int zmpo_shapedgemm(int dtype, int shape, const zmpo_task* tasks, const zmpo_uni
   if (dtype == ZMPO_F32)
     return dispatch_shaped<float>(shape, tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
   return set_error(ZMPO_ERR_ARG, "shapedgemm: unsupported dtype");
+}
+
+int zmpo_chaingemm(int dtype, int rb, int ncols_max, const zmpo_cunit* units, int64_t nunits,
+                   const zmpo_cgroup* groups, const zmpo_task* tasks, const zmpo_pair* pairs,
+                   const void* A, const void* B1, const void* B2, void* C, int beta,
+                   void* stream) {
+  if (nunits <= 0) return ZMPO_OK;
+  if (!units || !groups || !tasks || !pairs || !A || !B1 || !B2 || !C)
+    return set_error(ZMPO_ERR_ARG, "chaingemm: null pointer");
+  if (nunits > 0x7fffffffLL) return set_error(ZMPO_ERR_ARG, "chaingemm: too many units");
+  if (ncols_max < 1) return set_error(ZMPO_ERR_ARG, "chaingemm: ncols_max must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == ZMPO_F64) {
+    if (rb == 1) return launch_chain<double, 1>(ncols_max, units, nunits, groups, tasks, pairs, A, B1, B2, C, beta, s);
+    if (rb == 2) return launch_chain<double, 2>(ncols_max, units, nunits, groups, tasks, pairs, A, B1, B2, C, beta, s);
+  } else if (dtype == ZMPO_F32) {
+    if (rb == 1) return launch_chain<float, 1>(ncols_max, units, nunits, groups, tasks, pairs, A, B1, B2, C, beta, s);
+    if (rb == 2) return launch_chain<float, 2>(ncols_max, units, nunits, groups, tasks, pairs, A, B1, B2, C, beta, s);
+  }
+  return set_error(ZMPO_ERR_ARG, "chaingemm: unsupported dtype or row-block count");
+}
+
+int zmpo_streamgemm(int dtype, int shape, int kt, const zmpo_sunit* units, int64_t nunits,
+                    int upw, const zmpo_task* tasks, const zmpo_rgroup* groups,
+                    int64_t ngroups, const zmpo_pair* pairs, const void* A, const void* B,
+                    void* C, void* work, int beta, void* stream) {
+  if (nunits > 0 && (!units || !pairs || !A || !B || !C))
+    return set_error(ZMPO_ERR_ARG, "streamgemm: null pointer");
+  if (ngroups > 0 && (!groups || !work || !tasks))
+    return set_error(ZMPO_ERR_ARG, "streamgemm: null workspace");
+  if (upw < 1 || upw > MAXUPW) return set_error(ZMPO_ERR_ARG, "streamgemm: upw must be in [1, 64]");
+  if (nunits < 0 || ngroups < 0 || (nunits + upw - 1) / upw > 0x7fffffffLL ||
+      ngroups > 0x7fffffffLL)
+    return set_error(ZMPO_ERR_ARG, "streamgemm: bad unit or group count");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == ZMPO_F64)
+    return dispatch_stream<double>(shape, kt, units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+  if (dtype == ZMPO_F32)
+    return dispatch_stream<float>(shape, kt, units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+  return set_error(ZMPO_ERR_ARG, "streamgemm: unsupported dtype");
 }
 
 int zmpo_dots(int dtype, const void* X, int64_t ldx, int nx, const void* y, int64_t n,

@@ -75,6 +75,11 @@ def stream():
     return torch().cuda.current_stream().cuda_stream
 
 
+def stream_obj():
+    """The torch stream object whose handle stream() returns (kernel launch stream)."""
+    return torch().cuda.current_stream()
+
+
 def sync():
     torch().cuda.synchronize()
 

@@ -24,12 +24,16 @@ import numpy as np
 
 from . import _lib
 from . import device as dv
-from .qtensor import Program, QTensor, gather, run_program
+import os
+
+from .qtensor import ChainProgram, Program, QTensor, chain_structure, gather, run_chain, run_program
 
 ENV_PERM = (1, 0, 2)
 TOP1_PERM = (1, 0, 2, 3, 4)
 TOP3_PERM = (0, 3, 1, 2, 4)
 DEFAULT_BUDGET = 32 * 2 ** 30     # bytes of intermediates per op chunk
+# sigma stages 2-3 as one chained launch (intermediate top2 kept in LDS)
+FUSED = os.environ.get('ZMPO_FUSED', '1') != '0'
 
 
 def case(porb, isite, ncsite=1):
@@ -151,8 +155,10 @@ class SigmaProgram:
     """Compiled two-site sigma for one (site, psi template): per op chunk,
     four grouped-GEMM programs and the intermediate offsets."""
 
-    def __init__(self, mpo, isite, qtmp, qkey, lenv, renv, hpwts, budget=DEFAULT_BUDGET):
+    def __init__(self, mpo, isite, qtmp, qkey, lenv, renv, hpwts, budget=DEFAULT_BUDGET,
+                 fused=None):
         self.isite = isite
+        self.fused = FUSED if fused is None else fused
         self.ndim0 = qtmp.size_allowed
         jsite = isite + 1
         nops = mpo.nops
@@ -182,13 +188,24 @@ class SigmaProgram:
             plans[c] = [t.plan.lowered() for t in (t1, t2, t3, t4)] + [t1, t2, t3]
         self.plans = plans
         self.case_of = [case(mpo.opers[p], isite, 2) for p in range(nops)]
-        sz = [sum(plans[c][k].size_allowed for k in (4, 5, 6)) * 8 for c in self.case_of]
+        kept = (4, 6) if self.fused else (4, 5, 6)
+        sz = [sum(plans[c][k].size_allowed for k in kept) * 8 for c in self.case_of]
         self.chunks = _chunks(sz, budget)
         self.stages = []
+        self.chains = []
         self.inter = [0, 0, 0]
+        structs = {}
+        if self.fused:
+            for c, pl in plans.items():
+                t2, t3 = pl[5], pl[6]
+                nr = 1 << 24
+                mid_key = t2.blk_isyms[:, 0] * nr + t2.blk_isyms[:, 2]
+                out_key = t3.blk_isyms[:, 0] * nr + t3.blk_isyms[:, 1]
+                structs[c] = chain_structure(pl[1], pl[2], mid_key, out_key, t2.blk_off)
         for ch in self.chunks:
             offs = [0, 0, 0]
             parts = [[], [], [], []]
+            inst = []
             for p in ch:
                 pl = plans[self.case_of[p]]
                 sizes = [pl[4].size_allowed, pl[5].size_allowed, pl[6].size_allowed]
@@ -202,6 +219,7 @@ class SigmaProgram:
                 parts[1].append(_instantiate(pl[1], offs[0], wo, offs[1], 1.0, shift[1]))
                 parts[2].append(_instantiate(pl[2], offs[1], vo, offs[2], 1.0, shift[2]))
                 parts[3].append(_instantiate(pl[3], offs[2], ro, 0, float(hpwts[p]), 0))
+                inst.append((self.case_of[p], offs[0], wo, vo, offs[2]))
                 for k in range(3):
                     offs[k] += sizes[k]
             progs = []
@@ -214,28 +232,42 @@ class SigmaProgram:
             for k in range(4):
                 self.flops_stage[k] += progs[k].flops
             self.stages.append(progs)
+            self.chains.append(ChainProgram(structs, inst) if self.fused else None)
             self.inter = [max(a, b) for a, b in zip(self.inter, offs)]
         self.flops = int(self.flops_stage.sum())
         self._buffers = {}
 
-    def buffers(self, dtype):
+    def buffers(self, dtype, mid=None):
+        """Intermediate buffers [top1, top2, top3]; top2 is allocated only for
+        the unfused path (mid=True, default: not self.fused)."""
+        mid = (not self.fused) if mid is None else mid
         if dtype not in self._buffers:
-            self._buffers[dtype] = [dv.empty(n, dtype) for n in self.inter]
-        return self._buffers[dtype]
+            self._buffers[dtype] = [dv.empty(self.inter[0], dtype), None,
+                                    dv.empty(self.inter[2], dtype)]
+        b = self._buffers[dtype]
+        if mid and b[1] is None:
+            b[1] = dv.empty(self.inter[1], dtype)
+        return b
 
-    def apply(self, mpo, lenv, renv, psi, sigma, dtype=0):
+    def apply(self, mpo, lenv, renv, psi, sigma, dtype=0, fused=None):
         """sigma = H psi (device arrays of length ndim0, same dtype)."""
-        t1, t2, t3 = self.buffers(dtype)
+        fused = self.fused if fused is None else fused
+        t1, t2, t3 = self.buffers(dtype, mid=not fused)
         L, R = lenv.buffer(dtype), renv.buffer(dtype)
         Wi, Wj = mpo.buffer(self.isite, dtype), mpo.buffer(self.isite + 1, dtype)
         if not self.stages or all(st[3].ntasks == 0 for st in self.stages):
             sigma[:] = 0
             return sigma
+        if fused and not self.fused:
+            raise ValueError('SigmaProgram built without the chained stages (fused=False)')
         first = True
-        for st in self.stages:
+        for st, chain in zip(self.stages, self.chains):
             run_program(st[0], L, psi, t1, dtype)
-            run_program(st[1], t1, Wi, t2, dtype)
-            run_program(st[2], t2, Wj, t3, dtype)
+            if fused:
+                run_chain(chain, t1, Wi, Wj, t3, dtype)
+            else:
+                run_program(st[1], t1, Wi, t2, dtype)
+                run_program(st[2], t2, Wj, t3, dtype)
             if st[3].ntasks:
                 run_program(st[3], t3, R, sigma, dtype, beta=0 if first else 1)
                 first = False

@@ -558,6 +558,36 @@ def choose_rb(tasks):
     return max(rb for rb in util if util[rb] >= 0.85 * best)
 
 
+STREAM_KT = int(os.environ.get('ZMPO_KT', '32'))
+STREAM_UPW = int(os.environ.get('ZMPO_UPW', '0'))
+
+
+def _max_row_offset(lo, shi, slo, extent):
+    """Largest in-block row offset (r // lo) * shi + (r % lo) * slo, r < extent."""
+    lo = lo.astype(np.int64)
+    ext = extent.astype(np.int64)
+    hi = (ext - 1) // lo
+    last = np.minimum(ext, lo) - 1
+    return int(np.max(hi * np.abs(shi.astype(np.int64)) + last * np.abs(slo.astype(np.int64))))
+
+
+def stream_units(tasks, units):
+    """zmpo_sunit records: the units with their task geometry folded in."""
+    t = tasks[units['task']]
+    su = np.zeros(len(units), dtype=_lib.SUNIT_DTYPE)
+    for f in ('c_off', 'c_smhi', 'c_mlo', 'c_smlo', 'c_sn', 'M', 'N'):
+        su[f] = t[f]
+    for f in ('m0', 'n0', 'p_begin', 'p_end', 'slot'):
+        su[f] = units[f]
+    return su
+
+
+def stream_upw(nunits):
+    """Units per workgroup: long ranges for big launches (>= ~4096 workgroups
+    stay in flight), one unit each for small ones."""
+    return int(min(64, max(1, nunits // 4096)))
+
+
 class Program:
     """Task/pair arrays of one grouped-GEMM launch (uploaded on first use)."""
 
@@ -574,6 +604,27 @@ class Program:
                                 * pairs['K'].astype(np.int64))) if self.npairs else 0
         self.rb = choose_rb(tasks) if self.ntasks else 1
         self.shape = choose_shape(tasks, pairs) if self.ntasks else 0
+        self.kt = None    # stream kernel k-tile (None: library default)
+        self.upw = None   # stream kernel units per workgroup (None: by launch size)
+        # the stream kernel keeps in-block row offsets in 32 bits
+        self.small_blocks = bool(self.npairs == 0 or max(
+            _max_row_offset(pairs['a_mlo'], pairs['a_smhi'], pairs['a_smlo'], tasks['M'][rows]),
+            _max_row_offset(pairs['b_nlo'], pairs['b_snhi'], pairs['b_snlo'], tasks['N'][rows]))
+            < 2 ** 31 - 1)
+
+    def alg_bytes(self, elsize=8, beta=0):
+        """Algorithmic HBM bytes of one launch: every distinct A and B block
+        read once, every output block written once (and read once if beta)."""
+        if self.npairs == 0:
+            return 0
+        t, p = self.tasks_host, self.pairs_host
+        rows = np.repeat(np.arange(self.ntasks), (t['p_end'] - t['p_begin']).astype(np.int64))
+        a_sz = t['M'][rows].astype(np.int64) * p['K']
+        b_sz = t['N'][rows].astype(np.int64) * p['K']
+        _, ia = np.unique(p['a_off'], return_index=True)
+        _, ib = np.unique(p['b_off'], return_index=True)
+        c = int(np.sum(t['M'].astype(np.int64) * t['N']))
+        return elsize * (int(a_sz[ia].sum()) + int(b_sz[ib].sum()) + c * (2 if beta else 1))
 
     def device(self):
         if self._dev is None:
@@ -589,10 +640,17 @@ class Program:
             units, groups, nslots = make_schedule(self.tasks_host, self.pairs_host, self.rb,
                                                   reuse_key=self.reuse_key, tile=tile)
             size = self.rb * 256 if kernel == 'wave' else tile[0] * tile[1]
-            self._sched[kernel] = {'units': dv.upload_bytes(units), 'nunits': len(units),
-                                   'groups': dv.upload_bytes(groups), 'ngroups': len(groups),
-                                   'nslots': nslots, 'slot_size': size,
-                                   'units_host': units, 'groups_host': groups}
+            sc = {'nunits': len(units), 'groups': dv.upload_bytes(groups),
+                  'ngroups': len(groups), 'nslots': nslots, 'slot_size': size,
+                  'units_host': units, 'groups_host': groups}
+            if kernel == 'stream':
+                su = stream_units(self.tasks_host, units)
+                sc['units'] = dv.upload_bytes(su)
+                sc['kt'] = self.kt or STREAM_KT
+                sc['upw'] = self.upw or STREAM_UPW or stream_upw(len(su))
+            else:
+                sc['units'] = dv.upload_bytes(units)
+            self._sched[kernel] = sc
         return self._sched[kernel]
 
 
@@ -613,12 +671,15 @@ KERNEL = os.environ.get('ZMPO_KERNEL', 'shaped')
 def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None):
     """Launch one grouped block GEMM: C (=|+=) sum over pairs of A_pair B_pair.
 
-    kernel='shaped' (default) runs LDS-staged block-shaped workgroup tiles,
-    'wave' one wave per small tile, both with deterministic split-K;
+    kernel='stream' runs workgroups over ranges of block-shaped tiles as one
+    software pipeline, 'shaped' one LDS-staged tile per workgroup, 'wave' one
+    wave per small tile, all with deterministic split-K;
     'tile' runs the 64x64 LDS-tiled workgroup kernel."""
     kernel = KERNEL if kernel is None else kernel
     if isinstance(prog, dict):
         prog = Program(prog['tasks'], prog['pairs'])
+    if kernel == 'stream' and not prog.small_blocks:
+        kernel = 'shaped'
     if prog.ntasks == 0:
         return
     tasks, pairs = prog.device()
@@ -630,7 +691,12 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None)
         return
     sc = prog.schedule(kernel)
     work = _workspace(sc['nslots'] * sc['slot_size'], dtype) if sc['nslots'] else None
-    if kernel == 'wave':
+    if kernel == 'stream':
+        _lib.check(L.zmpo_streamgemm(dtype, prog.shape, sc['kt'], dv.ptr(sc['units']),
+                                     sc['nunits'], sc['upw'], dv.ptr(tasks), dv.ptr(sc['groups']),
+                                     sc['ngroups'], dv.ptr(pairs), dv.ptr(A), dv.ptr(B),
+                                     dv.ptr(C), dv.ptr(work), beta, st))
+    elif kernel == 'wave':
         _lib.check(L.zmpo_wavegemm(dtype, prog.rb, dv.ptr(tasks), dv.ptr(sc['units']),
                                    sc['nunits'], dv.ptr(sc['groups']), sc['ngroups'],
                                    dv.ptr(pairs), dv.ptr(A), dv.ptr(B), dv.ptr(C), dv.ptr(work),
@@ -640,6 +706,215 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None)
                                      sc['nunits'], dv.ptr(sc['groups']), sc['ngroups'],
                                      dv.ptr(pairs), dv.ptr(A), dv.ptr(B), dv.ptr(C),
                                      dv.ptr(work), beta, st))
+
+
+def chain_structure(first, second, mid_key, out_key, mid_off):
+    """Row-group structure of two chained plans (one operator case).
+
+    ``first`` lowers X = A B1 (its tasks are the intermediate blocks, in
+    block order, with c_off = mid_off), ``second`` lowers C = X B2 (pair
+    a_off = an intermediate block offset).  mid_key / out_key give the row
+    group of every intermediate / output block; the rows of blocks of one
+    group coincide.  Returns (groups, tasks, pairs) for zmpo_chaingemm with
+    offsets relative to each operand's base; groups without output blocks
+    are dropped (their intermediate is never read).
+    """
+    t1, p1 = first['tasks'], first['pairs']
+    t2, p2 = second['tasks'], second['pairs']
+    mid_key = np.asarray(mid_key, np.int64)
+    out_key = np.asarray(out_key, np.int64)
+    keys = np.unique(out_key)
+    col = np.zeros(len(t1), np.int64)
+    grp_of_mid = np.full(len(t1), -1, np.int64)
+    gi = np.searchsorted(keys, mid_key)
+    ok = (gi < len(keys)) & (keys[np.minimum(gi, len(keys) - 1)] == mid_key)
+    grp_of_mid[ok] = gi[ok]
+    # intermediate column offsets: cumulative widths inside each group
+    order1 = np.lexsort((np.arange(len(t1)), grp_of_mid))
+    order1 = order1[grp_of_mid[order1] >= 0]
+    g1 = grp_of_mid[order1]
+    w1 = t1['N'][order1].astype(np.int64)
+    csum = np.cumsum(w1) - w1
+    start = np.concatenate([[0], np.flatnonzero(np.diff(g1)) + 1]) if len(g1) else np.zeros(0, np.int64)
+    first_of = np.repeat(csum[start], np.diff(np.append(start, len(g1)))) if len(g1) else csum
+    col[order1] = csum - first_of
+    n1 = np.bincount(g1, minlength=len(keys))
+    ncols = np.bincount(g1, weights=w1, minlength=len(keys)).astype(np.int64)
+    grp_of_out = np.searchsorted(keys, out_key)
+    order2 = np.lexsort((np.arange(len(t2)), grp_of_out))
+    n2 = np.bincount(grp_of_out, minlength=len(keys))
+    # the intermediate block each second-stage pair reads
+    mid_idx = {int(o): j for j, o in enumerate(mid_off)}
+    pa = np.array([mid_idx[int(o)] for o in p2['a_off']], dtype=np.int64) if len(p2) \
+        else np.zeros(0, np.int64)
+    rows2 = np.repeat(np.arange(len(t2)), (t2['p_end'] - t2['p_begin']).astype(np.int64))
+    if len(pa) and np.any(grp_of_mid[pa] != grp_of_out[rows2]):
+        raise RuntimeError('chained plans: a second-stage pair reads another row group')
+    # tasks: per group its first-stage tasks, then its second-stage tasks
+    tb1 = np.concatenate([[0], np.cumsum(n1 + n2)[:-1]]).astype(np.int64)
+    tasks = np.zeros(int((n1 + n2).sum()), _lib.TASK_DTYPE)
+    pairs_parts = []
+    pcount = 0
+    new_t1 = np.empty(len(order1), np.int64)
+    pos1 = tb1[g1] + (np.arange(len(g1)) - (np.repeat(start, np.diff(np.append(start, len(g1))))
+                                              if len(g1) else 0))
+    new_t1[:] = pos1
+    g2 = grp_of_out[order2]
+    s2 = np.concatenate([[0], np.flatnonzero(np.diff(g2)) + 1]) if len(g2) else np.zeros(0, np.int64)
+    pos2 = tb1[g2] + n1[g2] + (np.arange(len(g2)) - np.repeat(s2, np.diff(np.append(s2, len(g2))))
+                               if len(g2) else 0)
+    src = np.empty(len(tasks), np.int64)
+    stage = np.zeros(len(tasks), np.int8)
+    src[pos1] = order1
+    src[pos2] = order2
+    stage[pos2] = 1
+    p2 = p2.copy()
+    if len(pa):
+        p2['a_off'] = col[pa]
+    for ti in range(len(tasks)):
+        j = int(src[ti])
+        if stage[ti] == 0:
+            tk = t1[j]
+            pp = p1[int(tk['p_begin']):int(tk['p_end'])]
+        else:
+            tk = t2[j]
+            pp = p2[int(tk['p_begin']):int(tk['p_end'])]
+        tasks[ti] = tk
+        if stage[ti] == 0:
+            tasks['c_off'][ti] = col[j]
+        tasks['p_begin'][ti] = pcount
+        pcount += len(pp)
+        tasks['p_end'][ti] = pcount
+        pairs_parts.append(pp)
+    pairs = np.concatenate(pairs_parts) if pairs_parts else np.zeros(0, _lib.PAIR_DTYPE)
+    groups = np.zeros(len(keys), _lib.CGROUP_DTYPE)
+    Mg = np.zeros(len(keys), np.int64)
+    Mg[g2] = t2['M'][order2]
+    if len(g1) and np.any(t1['M'][order1] != Mg[g1]):
+        raise RuntimeError('chained plans: blocks of one row group differ in rows')
+    groups['M'] = Mg
+    groups['ncols'] = ncols
+    groups['t1_begin'] = tb1
+    groups['t1_end'] = tb1 + n1
+    groups['t2_begin'] = tb1 + n1
+    groups['t2_end'] = tb1 + n1 + n2
+    return groups, tasks, pairs
+
+
+class ChainProgram:
+    """One zmpo_chaingemm launch: the row-group structures of several cases
+    instantiated for many operators (units = row tiles of 16*rb rows)."""
+
+    def __init__(self, structures, instances, rb=1):
+        """structures: {case: (groups, tasks, pairs)}; instances: list of
+        (case, a_base, b1_base, b2_base, c_base)."""
+        self.rb = rb
+        self._structs, self._inst = structures, instances
+        tm = 16 * rb
+        gparts, tparts, pparts = [], [], []
+        gbase, tbase, pbase = {}, 0, 0
+        self._gspan = {}
+        ng = 0
+        for c in sorted(structures):
+            g, t, p = structures[c]
+            g = g.copy()
+            t = t.copy()
+            for f in ('t1_begin', 't1_end', 't2_begin', 't2_end'):
+                g[f] += tbase
+            t['p_begin'] += pbase
+            t['p_end'] += pbase
+            gbase[c] = ng
+            self._gspan[c] = (ng, len(g))
+            ng += len(g)
+            tbase += len(t)
+            pbase += len(p)
+            gparts.append(g)
+            tparts.append(t)
+            pparts.append(p)
+        self.groups_host = np.concatenate(gparts) if gparts else np.zeros(0, _lib.CGROUP_DTYPE)
+        self.tasks_host = np.concatenate(tparts) if tparts else np.zeros(0, _lib.TASK_DTYPE)
+        self.pairs_host = np.concatenate(pparts) if pparts else np.zeros(0, _lib.PAIR_DTYPE)
+        self.ncols_max = int(max(1, self.groups_host['ncols'].max())) if ng else 1
+        uparts = []
+        self.flops = 0
+        t, p = self.tasks_host, self.pairs_host
+        prow = np.repeat(np.arange(len(t)), (t['p_end'] - t['p_begin']).astype(np.int64))
+        pfl = 2 * t['M'][prow].astype(np.int64) * t['N'][prow] * p['K'].astype(np.int64)
+        tfl = np.bincount(prow, weights=pfl, minlength=len(t)) if len(p) else np.zeros(len(t))
+        self._tfl = tfl
+        self.instance_cases = [c for c, *_ in instances]
+        for c, a0, b10, b20, c0 in instances:
+            g, tt, _ = structures[c]
+            if len(g) == 0:
+                continue
+            ntile = np.maximum((g['M'].astype(np.int64) + tm - 1) // tm, 1)
+            gi = np.repeat(np.arange(len(g)), ntile)
+            m0 = (np.arange(int(ntile.sum())) - np.repeat(np.cumsum(ntile) - ntile, ntile)) * tm
+            u = np.zeros(len(gi), _lib.CUNIT_DTYPE)
+            u['a_base'], u['b1_base'], u['b2_base'], u['c_base'] = a0, b10, b20, c0
+            u['group'] = gi + gbase[c]
+            u['m0'] = m0
+            uparts.append(u)
+            gl = gbase[c]
+            self.flops += int(sum(tfl[self.groups_host['t1_begin'][gl]:self.groups_host['t2_end'][gl + len(g) - 1]]))
+        self.units_host = np.concatenate(uparts) if uparts else np.zeros(0, _lib.CUNIT_DTYPE)
+        self.nunits = len(self.units_host)
+        self._dev = None
+
+    def with_rb(self, rb):
+        """The same launch with 16*rb-row tiles."""
+        return self if rb == self.rb else ChainProgram(self._structs, self._inst, rb)
+
+    def alg_bytes(self, elsize=8):
+        """Algorithmic HBM bytes of one launch: the first-stage A blocks and
+        the output blocks moved once, and every operator's B1/B2 blocks read
+        once; the intermediate stays on chip."""
+        if self.nunits == 0:
+            return 0
+        t, p, g = self.tasks_host, self.pairs_host, self.groups_host
+        cols = np.zeros(len(g), np.int64)
+        for gi, gg in enumerate(g):
+            if gg['t1_end'] > gg['t1_begin']:
+                pp = p[t['p_begin'][gg['t1_begin']]:t['p_end'][gg['t1_end'] - 1]]
+                _, ia = np.unique(pp['a_off'], return_index=True)
+                cols[gi] += int(pp['K'][ia].sum())
+            cols[gi] += int(t['N'][gg['t2_begin']:gg['t2_end']].sum())
+        u = self.units_host
+        rows = np.minimum(16 * self.rb, g['M'][u['group']].astype(np.int64) - u['m0'])
+        tot = int(np.sum(rows * cols[u['group']]))
+        prow = np.repeat(np.arange(len(t)), (t['p_end'] - t['p_begin']).astype(np.int64))
+        bsz = t['N'][prow].astype(np.int64) * p['K']
+        for c in self.instance_cases:
+            gl, gn = self._gspan[c]
+            if gn == 0:
+                continue
+            lo = t['p_begin'][g['t1_begin'][gl]]
+            hi = t['p_end'][g['t2_end'][gl + gn - 1] - 1]
+            stg = np.zeros(hi - lo, np.int64)
+            for gg in g[gl:gl + gn]:
+                stg[t['p_begin'][gg['t2_begin']] - lo:t['p_end'][gg['t2_end'] - 1] - lo] = 1 \
+                    if gg['t2_end'] > gg['t2_begin'] else 0
+            key = p['b_off'][lo:hi] * 2 + stg
+            _, ib = np.unique(key, return_index=True)
+            tot += int(bsz[lo:hi][ib].sum())
+        return elsize * tot
+
+    def device(self):
+        if self._dev is None:
+            self._dev = tuple(dv.upload_bytes(x) for x in (self.units_host, self.groups_host,
+                                                           self.tasks_host, self.pairs_host))
+        return self._dev
+
+
+def run_chain(prog, A, B1, B2, C, dtype=_lib.F64, beta=0, stream=None):
+    """Launch one zmpo_chaingemm: C (=|+=) (A B1) B2 over the program's row tiles."""
+    if prog.nunits == 0:
+        return
+    u, g, t, p = prog.device()
+    st = dv.stream() if stream is None else stream
+    _lib.check(_lib.lib().zmpo_chaingemm(dtype, prog.rb, prog.ncols_max, dv.ptr(u), prog.nunits,
+                                         dv.ptr(g), dv.ptr(t), dv.ptr(p), dv.ptr(A), dv.ptr(B1),
+                                         dv.ptr(B2), dv.ptr(C), beta, st))
 
 
 def gather(dst, src, idx, dtype_dst=_lib.F64, dtype_src=_lib.F64):
