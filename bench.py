@@ -90,6 +90,8 @@ def time_stages(prog, mpo, L, R, psi, sig, dtype, reps=3):
     t1, t2, t3 = prog.buffers(dtype, mid=not fused)
     Lb, Rb = L.buffer(dtype), R.buffer(dtype)
     Wi, Wj = mpo.buffer(prog.isite, dtype), mpo.buffer(prog.isite + 1, dtype)
+    if fused:
+        Wpi, Wpj = mpo.padded_buffer(prog.isite, dtype), mpo.padded_buffer(prog.isite + 1, dtype)
     elsize = 8 if dtype == 0 else 4
     if fused:
         names = ['L*psi', 'W[i]*W[i+1] chained', '*R']
@@ -102,7 +104,7 @@ def time_stages(prog, mpo, L, R, psi, sig, dtype, reps=3):
         for st, ch in zip(prog.stages, prog.chains):
             if fused:
                 ops = [(lambda: run_program(st[0], Lb, psi, t1, dtype, beta=0), st[0]),
-                       (lambda: run_chain(ch, t1, Wi, Wj, t3, dtype), ch),
+                       (lambda: run_chain(ch, t1, Wpi, Wpj, t3, dtype), ch),
                        (lambda: run_program(st[3], t3, Rb, sig, dtype, beta=0), st[3])]
             else:
                 ops = [(lambda: run_program(st[0], Lb, psi, t1, dtype, beta=0), st[0]),

@@ -163,29 +163,44 @@ int zmpo_streamgemm(int dtype, int shape, int kt, const zmpo_sunit* units, int64
  * the middle sigma stages of HVecQt (mpo_dmrg_kernelQt.py:277-296: top1 x
  * W[i] -> top2, top2 x W[i+1] -> top3).  All blocks of one row group share
  * their M rows (the bra and right-bond sectors), so for a tile of 16*rb rows
- * the group's whole intermediate (every first-stage output block, `ncols`
+ * the group's distinct A blocks (`kcols` columns) are staged into LDS once
+ * and its whole intermediate (every first-stage output block, `ncols`
  * columns) stays in LDS and feeds the second stage; it never touches HBM.
+ *   A blocks [ab_begin, ab_end): zmpo_cblock, offsets relative to a_base;
  *   first-stage tasks  [t1_begin, t1_end): c_off = intermediate column
- *     offset, N = its width; pairs: a_off/A maps relative to a_base, b_off
- *     relative to b1_base.
+ *     offset, N = its width (ceil4(N) columns reserved); pairs: a_off =
+ *     A-tile column offset (A maps unused), b_off relative to b1_base;
  *   second-stage tasks [t2_begin, t2_end): c_off and C maps relative to
- *     c_base; pairs: a_off = intermediate column offset (A maps unused),
+ *     c_base (c_mlo == rsplit); pairs: a_off = intermediate column offset,
  *     b_off relative to b2_base.
- * One workgroup per zmpo_cunit (rows [m0, m0 + 16*rb) of group `group`). */
+ *   B blocks are zero-padded row-major matrices [ceil4(K)][ceil32(N)] with
+ *   row stride b_sk (so operand loads need no bounds tests); pair scales
+ *   are 1.
+ * One workgroup per zmpo_cunit (rows [m0, m0 + 16*rb) of group `group`);
+ * each output element is written once (no atomics). */
 typedef struct zmpo_cgroup {
-  int32_t M, ncols, t1_begin, t1_end, t2_begin, t2_end;
+  int32_t M, ncols, kcols, t1_begin, t1_end, t2_begin, t2_end, ab_begin, ab_end, rsplit;
 } zmpo_cgroup;
+
+/* One A block of a row group: rows m map to (m / rsplit) * a_smhi +
+ * (m % rsplit) * a_smlo (a_mlo == the group's rsplit), column k to k * a_sk;
+ * staged at A-tile columns [col, col + ceil4(K)) (padding columns zero). */
+typedef struct zmpo_cblock {
+  int64_t a_off, a_smhi, a_smlo, a_sk;
+  int32_t K, col, a_mlo, pad;
+} zmpo_cblock;
 
 typedef struct zmpo_cunit {
   int64_t a_base, b1_base, b2_base, c_base;
   int32_t group, m0;
 } zmpo_cunit;
 
-/* rb in {1, 2}; ncols_max >= every group's ncols (sizes the LDS). */
-int zmpo_chaingemm(int dtype, int rb, int ncols_max, const zmpo_cunit* units, int64_t nunits,
-                   const zmpo_cgroup* groups, const zmpo_task* tasks, const zmpo_pair* pairs,
-                   const void* A, const void* B1, const void* B2, void* C, int beta,
-                   void* stream);
+/* rb in {1, 2}; kcols_max / ncols_max >= every group's kcols / ncols (they
+ * size the LDS: (kcols_max + ncols_max) * (16*rb + 1) elements <= 160 KiB). */
+int zmpo_chaingemm(int dtype, int rb, int kcols_max, int ncols_max, const zmpo_cunit* units,
+                   int64_t nunits, const zmpo_cgroup* groups, const zmpo_cblock* blocks,
+                   const zmpo_task* tasks, const zmpo_pair* pairs, const void* A, const void* B1,
+                   const void* B2, void* C, int beta, void* stream);
 
 /* out[j] = sum_i X[j*ldx + i] * y[i] for j < nx (device out, float64;
  * work holds zmpo_dots_workspace(nx) doubles).  Fixed reduction order. */

@@ -200,37 +200,41 @@ def emulate_stream(prog, A, B, C, kt, upw, beta=0):
 
 
 def emulate_chain(prog, A, B1, B2, C, beta=0):
-    """Model of zmpo_chaingemm: per unit, the group's intermediate columns
-    from the first-stage tasks, then the output blocks from it."""
+    """Model of zmpo_chaingemm: per unit, the group's A tile (staged A
+    blocks), the intermediate columns from the first-stage tasks, then the
+    output blocks from it."""
     A, B1, B2, C = _np(A), _np(B1), _np(B2), _np(C)
-    t, p, g = prog.tasks_host, prog.pairs_host, prog.groups_host
+    t, p, g, bl = prog.tasks_host, prog.pairs_host, prog.groups_host, prog.blocks_host
     tm = 16 * prog.rb
-    for u in prog.units_host:
-        gg = g[u['group']]
-        mv = min(tm, int(gg['M']) - int(u['m0']))
-        m = int(u['m0']) + np.arange(mv)
-        mid = np.zeros((mv, max(int(gg['ncols']), 1)))
-        for ti in range(int(gg['t1_begin']), int(gg['t1_end'])):
-            tk = t[ti]
-            n = np.arange(int(tk['N']))
-            acc = np.zeros((mv, len(n)))
-            for q in p[int(tk['p_begin']):int(tk['p_end'])]:
-                k = np.arange(int(q['K']))
-                ai = u['a_base'] + q['a_off'] + (m // q['a_mlo']) * q['a_smhi'] + (m % q['a_mlo']) * q['a_smlo']
-                bi = u['b1_base'] + q['b_off'] + (n // q['b_nlo']) * q['b_snhi'] + (n % q['b_nlo']) * q['b_snlo']
-                acc += q['scale'] * (A[ai[:, None] + k[None, :] * q['a_sk']] @ B1[k[:, None] * q['b_sk'] + bi[None, :]])
-            mid[:, int(tk['c_off']):int(tk['c_off']) + len(n)] = acc
-        for ti in range(int(gg['t2_begin']), int(gg['t2_end'])):
-            tk = t[ti]
-            n = np.arange(int(tk['N']))
-            acc = np.zeros((mv, len(n)))
-            for q in p[int(tk['p_begin']):int(tk['p_end'])]:
-                k = np.arange(int(q['K']))
-                bi = u['b2_base'] + q['b_off'] + (n // q['b_nlo']) * q['b_snhi'] + (n % q['b_nlo']) * q['b_snlo']
-                acc += q['scale'] * (mid[:, int(q['a_off']) + k] @ B2[k[:, None] * q['b_sk'] + bi[None, :]])
-            ci = (u['c_base'] + tk['c_off'] + (m // tk['c_mlo']) * tk['c_smhi']
-                  + (m % tk['c_mlo']) * tk['c_smlo'])[:, None] + n[None, :] * tk['c_sn']
-            C[ci] = acc if beta == 0 else C[ci] + acc
+    for l_ in prog.launches:
+        for u in l_['units']:
+            gg = g[u['group']]
+            assert gg['kcols'] <= l_['kmax'] and gg['ncols'] <= l_['nmax']
+            mv = min(tm, int(gg['M']) - int(u['m0']))
+            m = int(u['m0']) + np.arange(mv)
+            atile = np.zeros((mv, max(int(gg['kcols']), 1)))
+            for bk in bl[int(gg['ab_begin']):int(gg['ab_end'])]:
+                k = np.arange(int(bk['K']))
+                ai = u['a_base'] + bk['a_off'] + (m // bk['a_mlo']) * bk['a_smhi'] + (m % bk['a_mlo']) * bk['a_smlo']
+                atile[:, int(bk['col']) + k] = A[ai[:, None] + k[None, :] * bk['a_sk']]
+            mid = np.zeros((mv, max(int(gg['ncols']), 1)))
+            for lo, hi, src, B, base, to_mid in (
+                    (gg['t1_begin'], gg['t1_end'], atile, B1, u['b1_base'], True),
+                    (gg['t2_begin'], gg['t2_end'], mid, B2, u['b2_base'], False)):
+                for ti in range(int(lo), int(hi)):
+                    tk = t[ti]
+                    n = np.arange(int(tk['N']))
+                    acc = np.zeros((mv, len(n)))
+                    for q in p[int(tk['p_begin']):int(tk['p_end'])]:
+                        k = np.arange(int(q['K']))
+                        bi = base + q['b_off'] + (n // q['b_nlo']) * q['b_snhi'] + (n % q['b_nlo']) * q['b_snlo']
+                        acc += q['scale'] * (src[:, int(q['a_off']) + k] @ B[k[:, None] * q['b_sk'] + bi[None, :]])
+                    if to_mid:
+                        mid[:, int(tk['c_off']):int(tk['c_off']) + len(n)] = acc
+                    else:
+                        ci = (u['c_base'] + tk['c_off'] + (m // tk['c_mlo']) * tk['c_smhi']
+                              + (m % tk['c_mlo']) * tk['c_smlo'])[:, None] + n[None, :] * tk['c_sn']
+                        C[ci] = acc if beta == 0 else C[ci] + acc
 
 
 def _gather(dst, src, idx, *a):

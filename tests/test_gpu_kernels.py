@@ -193,16 +193,17 @@ def test_chaingemm_real_plan_matches_two_stages(gpu, rb, dtype):
     for st, ch in zip(sp.stages, sp.chains):
         ch = ch.with_rb(rb)
         n1 = int(st[1].pairs_host['a_off'].max() + 5000)
-        nw = int(max(st[1].pairs_host['b_off'].max(), st[2].pairs_host['b_off'].max()) + 5000)
         n2 = int(st[1].tasks_host['c_off'].max() + 5000)
         n3 = int(st[2].tasks_host['c_off'].max() + 5000)
-        A, Wi, Wj = rng.standard_normal(n1), rng.standard_normal(nw), rng.standard_normal(nw)
+        A = rng.standard_normal(n1)
+        Wi, Wj = case5.mpo.host[isite], case5.mpo.host[isite + 1]
+        Pi, Pj = case5.mpo.padded(isite)[0], case5.mpo.padded(isite + 1)[0]
         mid = np.zeros(n2)
         emulate(st[1], A, Wi, mid)
         want = np.zeros(n3)
         emulate(st[2], mid, Wj, want)
         C = dv.zeros(n3, dtype)
-        pq.run_chain(ch, dv.upload(A, dtype), dv.upload(Wi, dtype), dv.upload(Wj, dtype), C, dtype)
+        pq.run_chain(ch, dv.upload(A, dtype), dv.upload(Pi, dtype), dv.upload(Pj, dtype), C, dtype)
         tol = 1e-12 if dtype == 0 else 2e-5
         err = np.abs(dv.download(C).astype(np.float64) - want).max()
         assert err < tol * max(1.0, np.abs(want).max()), (rb, dtype, err)

@@ -189,15 +189,16 @@ def test_chain_program_matches_two_stages(rb):
     for st, ch in zip(sp.stages, sp.chains):
         ch = ch.with_rb(rb)
         n1 = int(st[1].pairs_host['a_off'].max() + 5000)
-        nw = int(max(st[1].pairs_host['b_off'].max(), st[2].pairs_host['b_off'].max()) + 5000)
         n2 = int(st[1].tasks_host['c_off'].max() + 5000)
         n3 = int(st[2].tasks_host['c_off'].max() + 5000)
-        A, Wi, Wj = rng.standard_normal(n1), rng.standard_normal(nw), rng.standard_normal(nw)
+        A = rng.standard_normal(n1)
+        Wi, Wj = case5.mpo.host[isite], case5.mpo.host[isite + 1]
+        Pi, Pj = case5.mpo.padded(isite)[0], case5.mpo.padded(isite + 1)[0]
         mid = np.zeros(n2)
         emulate(st[1], A, Wi, mid)
         want = np.zeros(n3)
         emulate(st[2], mid, Wj, want)
         got = np.zeros(n3)
-        emulate_chain(ch, A, Wi, Wj, got)
+        emulate_chain(ch, A, Pi, Pj, got)
         assert np.abs(got - want).max() < 1e-11 * max(1.0, np.abs(want).max())
         assert ch.flops == st[1].flops + st[2].flops

@@ -26,8 +26,11 @@ UNIT_DTYPE = np.dtype([('task', '<i4'), ('m0', '<i4'), ('n0', '<i4'), ('p_begin'
                        ('p_end', '<i4'), ('slot', '<i4'), ('pad0', '<i4'), ('pad1', '<i4')])
 RGROUP_DTYPE = np.dtype([('task', '<i4'), ('m0', '<i4'), ('n0', '<i4'), ('slot_begin', '<i4'),
                          ('nslots', '<i4'), ('pad', '<i4')])
-CGROUP_DTYPE = np.dtype([('M', '<i4'), ('ncols', '<i4'), ('t1_begin', '<i4'), ('t1_end', '<i4'),
-                         ('t2_begin', '<i4'), ('t2_end', '<i4')])
+CGROUP_DTYPE = np.dtype([('M', '<i4'), ('ncols', '<i4'), ('kcols', '<i4'), ('t1_begin', '<i4'),
+                         ('t1_end', '<i4'), ('t2_begin', '<i4'), ('t2_end', '<i4'),
+                         ('ab_begin', '<i4'), ('ab_end', '<i4'), ('rsplit', '<i4')])
+CBLOCK_DTYPE = np.dtype([('a_off', '<i8'), ('a_smhi', '<i8'), ('a_smlo', '<i8'), ('a_sk', '<i8'),
+                         ('K', '<i4'), ('col', '<i4'), ('a_mlo', '<i4'), ('pad', '<i4')])
 CUNIT_DTYPE = np.dtype([('a_base', '<i8'), ('b1_base', '<i8'), ('b2_base', '<i8'),
                         ('c_base', '<i8'), ('group', '<i4'), ('m0', '<i4')])
 SUNIT_DTYPE = np.dtype([('c_off', '<i8'), ('c_smhi', '<i8'), ('c_mlo', '<i4'), ('c_smlo', '<i4'),
@@ -65,7 +68,8 @@ def lib():
     L.zmpo_blockgemm.argtypes = [i32, vp, i64, vp, i64, vp, vp, vp, i32, vp]
     L.zmpo_wavegemm.argtypes = [i32, i32, vp, vp, i64, vp, i64, vp, vp, vp, vp, vp, i32, vp]
     L.zmpo_shapedgemm.argtypes = [i32, i32, vp, vp, i64, vp, i64, vp, vp, vp, vp, vp, i32, vp]
-    L.zmpo_chaingemm.argtypes = [i32, i32, i32, vp, i64, vp, vp, vp, vp, vp, vp, vp, i32, vp]
+    L.zmpo_chaingemm.argtypes = [i32, i32, i32, i32, vp, i64, vp, vp, vp, vp, vp, vp, vp, vp,
+                                 i32, vp]
     L.zmpo_streamgemm.argtypes = [i32, i32, i32, vp, i64, i32, vp, vp, i64, vp, vp, vp, vp, vp,
                                   i32, vp]
     L.zmpo_dots.argtypes = [i32, vp, i64, i32, vp, i64, vp, vp, vp]

@@ -39,6 +39,12 @@ static_assert(sizeof(zmpo_unit) == 32, "zmpo_unit layout");
 static_assert(sizeof(zmpo_rgroup) == 24, "zmpo_rgroup layout");
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Index of the calling wave in its workgroup, known to the compiler as
+// wave-uniform (scalar registers, scalar branches and descriptor loads).
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
 template <typename T>
@@ -227,7 +233,7 @@ __global__ __launch_bounds__(256) void wave_gemm_kernel(
   typedef typename Acc<T>::type accv;
   __shared__ T stage[4][RB * 256];
   const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+  const int wv = wave_id();
   const int64_t blk = xcd_logical_block(blockIdx.x, gridDim.x);
   const int64_t u = blk * 4 + wv;
   const bool active = u < nunits;  // wave-uniform
@@ -378,6 +384,9 @@ struct TileLoader {
     ks = sk;
     K = K_;
     scale = sc;
+    // single-level row maps (mlo >= rmax: every row in the low index) need
+    // no division; the test is uniform
+    const bool one = mlo >= rmax;
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
       const int e = tid + 256 * i;
@@ -385,7 +394,12 @@ struct TileLoader {
       kk[i] = kmaj ? e % KT : e / ROWS;
       rr[i] = kmaj ? e / KT : e % ROWS;
       const int r = r0 + rr[i];
-      roff[i] = r < rmax ? (int64_t)(r / mlo) * shi + (int64_t)(r % mlo) * slo : (int64_t)-1;
+      int64_t o;
+      if (one)
+        o = (int64_t)r * slo;
+      else
+        o = (int64_t)(r / mlo) * shi + (int64_t)(r % mlo) * slo;
+      roff[i] = r < rmax ? o : (int64_t)-1;
     }
   }
   __device__ __forceinline__ void load(T (*dst)[ROWS + 1], int k0) const {
@@ -414,7 +428,7 @@ __global__ __launch_bounds__(256) void shaped_gemm_kernel(
   __shared__ T Bs[2][KT][TNv + 1];
   __shared__ T stage[TMv * TNv];
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6;
+  const int lane = tid & 63, wv = wave_id();
   const int lr = lane & 15, lk = lane >> 4;
   const int64_t u = xcd_logical_block(blockIdx.x, gridDim.x);
   if (u >= nunits) return;  // workgroup-uniform
@@ -496,12 +510,15 @@ __global__ __launch_bounds__(256) void shaped_gemm_kernel(
   }
   const int mv = min(TMv, tk.M - un.m0);
   const int nv = min(TNv, tk.N - un.n0);
-  for (int e = tid; e < mv * nv; e += 256) {
-    const int row = e / nv, col = e % nv;
+  const bool one = tk.c_mlo >= tk.M;  // single-level row map: no division
+  for (int e = tid; e < TMv * TNv; e += 256) {
+    const int row = e / TNv, col = e % TNv;
+    if (row >= mv || col >= nv) continue;
     const int mi = un.m0 + row, ni = un.n0 + col;
-    const int64_t off = tk.c_off + (int64_t)(mi / tk.c_mlo) * tk.c_smhi +
-                        (int64_t)(mi % tk.c_mlo) * tk.c_smlo + (int64_t)ni * tk.c_sn;
-    const T v = stage[row * TNv + col];
+    const int64_t ro = one ? (int64_t)mi * tk.c_smlo
+                           : (int64_t)(mi / tk.c_mlo) * tk.c_smhi + (int64_t)(mi % tk.c_mlo) * tk.c_smlo;
+    const int64_t off = tk.c_off + ro + (int64_t)ni * tk.c_sn;
+    const T v = stage[e];
     C[off] = beta ? C[off] + v : v;
   }
 }
@@ -516,11 +533,12 @@ __global__ __launch_bounds__(256) void shaped_reduce_kernel(
   const zmpo_task tk = tasks[gr.task];
   const int mv = min(TMv, tk.M - gr.m0);
   const int nv = min(TNv, tk.N - gr.n0);
-  for (int e = threadIdx.x; e < mv * nv; e += blockDim.x) {
-    const int row = e / nv, col = e % nv;
+  for (int e = threadIdx.x; e < TMv * TNv; e += blockDim.x) {
+    const int row = e / TNv, col = e % TNv;
+    if (row >= mv || col >= nv) continue;
     T s = 0;
     for (int q = 0; q < gr.nslots; ++q)
-      s += work[(int64_t)(gr.slot_begin + q) * (TMv * TNv) + row * TNv + col];
+      s += work[(int64_t)(gr.slot_begin + q) * (TMv * TNv) + e];
     const int mi = gr.m0 + row, ni = gr.n0 + col;
     const int64_t off = tk.c_off + (int64_t)(mi / tk.c_mlo) * tk.c_smhi +
                         (int64_t)(mi % tk.c_mlo) * tk.c_smlo + (int64_t)ni * tk.c_sn;
@@ -640,7 +658,7 @@ __global__ __launch_bounds__(256) void stream_gemm_kernel(
   __shared__ zmpo_sunit su[MAXUPW];
   T(*As)[TMv + 1] = reinterpret_cast<T(*)[TMv + 1]>(Abuf);
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6;
+  const int lane = tid & 63, wv = wave_id();
   const int lr = lane & 15, lk = lane >> 4;
   const int64_t u0 = xcd_logical_block(blockIdx.x, gridDim.x) * (int64_t)upw;
   if (u0 >= nunits) return;  // workgroup-uniform
@@ -831,24 +849,205 @@ int dispatch_stream(int shape, int kt, const zmpo_sunit* units, int64_t nunits, 
 }
 
 // ------------------------------------------------------------- chained GEMMs
-// One workgroup = TM (= 16*RB) rows of one row group.  Stage 1: each wave
-// takes first-stage tasks round-robin, accumulates its 16-column MFMA tiles
-// over the task's pairs (A rows from HBM, 16 consecutive rows per 128-byte
-// segment; B from the small MPO blocks) and writes them to the LDS
-// intermediate mid[col][row].  Stage 2 reads its A operand from mid and
-// writes the output blocks.  Rows past the group's M are loaded as zeros.
+// One workgroup = TM (= 16*RB) rows of one row group.
+//  stage 0: the group's distinct first-stage A blocks (TM rows each) are
+//           copied into the LDS tile atile[col][row] -- one wave per block,
+//           8 independent loads per lane in flight, 16 consecutive rows per
+//           128-byte segment; padding columns (up to a multiple of 4) get 0;
+//  stage 1: waves take first-stage tasks round-robin; each 16x32 MFMA tile
+//           accumulates over the task's pairs with A from atile and B (MPO
+//           blocks, zero-padded to [ceil4(K)][ceil32(N)], L2-resident) from
+//           global memory; the result goes to mid[col][row];
+//  stage 2: the same with A from mid, results written to C.
+// Every lane's rows are fixed for the whole unit, so the row maps (one
+// division by the group's row split) are computed once per unit; padded
+// operands make the inner loop free of bounds tests.  Rows past the group's
+// M are zeros in atile and are not written to C.
 extern __shared__ __align__(16) unsigned char chain_lds[];
+#ifndef CHAIN_KS
+#define CHAIN_KS 4  // k-steps (of 4) whose operands are loaded before their MFMAs
+#endif
+
+template <typename T, int RB>
+struct ChainSlice {
+  T a[CHAIN_KS][RB];
+  T b[CHAIN_KS][2];
+};
+
+// The fields of a pair that the chained stages use (uniform per wave).
+struct ChainPair {
+  int64_t b_off;
+  int K, c, ldb;
+  __device__ __forceinline__ void get(const zmpo_pair* __restrict__ pairs, int p) {
+    const zmpo_pair& q = pairs[p];
+    b_off = q.b_off;
+    K = q.K;
+    c = (int)q.a_off;
+    ldb = q.b_sk;
+  }
+};
+
+// Operands of the k-steps [k0, k0 + 4*CHAIN_KS) of one pair, 32-column tile
+// n0.  Only k-steps that start below K are loaded (uniform test); lanes past
+// K inside the last k-step read the zero padding of both operands.
+template <typename T, int RB>
+__device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPair& pr, int n0,
+                                           int k0, const T* __restrict__ src,
+                                           const T* __restrict__ Bb, int lr, int lk) {
+  constexpr int LD = 16 * RB + 1;
+  const T* __restrict__ b = Bb + pr.b_off + (int64_t)(k0 + lk) * pr.ldb + n0 + lr;
+  const T* __restrict__ a = src + (pr.c + k0 + lk) * LD + lr;
+#pragma unroll
+  for (int s = 0; s < CHAIN_KS; ++s) {
+    if (k0 + 4 * s < pr.K) {
+      sl.b[s][0] = b[4 * s * pr.ldb];
+      sl.b[s][1] = b[4 * s * pr.ldb + 16];
+#pragma unroll
+      for (int i = 0; i < RB; ++i) sl.a[s][i] = a[4 * s * LD + i * 16];
+    }
+  }
+}
+
+// One stage: tasks [t0, t1) of the group, A operand columns in src (LDS,
+// stride LD), B from global.  TO_LDS: write into dst LDS columns (task c_off
+// = first column; ceil4(N) columns), else into C with the task's maps, rows
+// split by the unit's precomputed (mb, mr).  Each wave walks its tasks
+// (t0 + wave, step 4) as one flat sequence of items -- an item is a
+// 4*CHAIN_KS-deep k slice of one pair for one 32-column tile -- and loads
+// the operands of item i+1 (possibly of the next pair or task) while the
+// MFMAs of item i run.
+template <typename T, int RB, bool TO_LDS>
+__device__ __forceinline__ void chain_stage(
+    int t0, int t1, const zmpo_task* __restrict__ tasks, const zmpo_pair* __restrict__ pairs,
+    const T* __restrict__ src, const T* __restrict__ Bb, T* __restrict__ dst,
+    T* __restrict__ C, int64_t c_base, const int (&mb)[RB][4], const int (&mr)[RB][4],
+    const bool (&rok)[RB][4], int beta, int wv, int lr, int lk) {
+  typedef typename Acc<T>::type accv;
+  constexpr int LD = 16 * RB + 1;
+  constexpr int KD = 4 * CHAIN_KS;
+  int t = t0 + wv;
+  if (t >= t1) return;  // wave-uniform
+  int tN = tasks[t].N, tpb = tasks[t].p_begin, tpe = tasks[t].p_end;
+  int n0 = 0, p = tpb, k0 = 0;
+  ChainPair pr;
+  pr.K = 0;
+  ChainSlice<T, RB> cur, nxt;
+  int cks = 0;
+  if (p < tpe) {
+    pr.get(pairs, p);
+    chain_load<T, RB>(cur, pr, n0, k0, src, Bb, lr, lk);
+    cks = min(CHAIN_KS, (pr.K - k0 + 3) / 4);
+  }
+  accv acc[RB][2];
+#pragma unroll
+  for (int i = 0; i < RB; ++i) acc[i][0] = acc[i][1] = accv{0, 0, 0, 0};
+  while (true) {
+    // advance the cursor (t, n0, p, k0) to the next item
+    int nt = t, nn0 = n0, np = p, nk0 = k0;
+    int nN = tN, npb = tpb, npe = tpe;
+    if (np < npe && nk0 + KD < pr.K) {
+      nk0 += KD;
+    } else {
+      nk0 = 0;
+      ++np;
+      if (np >= npe) {
+        np = npb;
+        nn0 += 32;
+        if (nn0 >= nN) {
+          nn0 = 0;
+          nt += 4;
+          if (nt < t1) {
+            nN = tasks[nt].N;
+            npb = tasks[nt].p_begin;
+            npe = tasks[nt].p_end;
+            np = npb;
+          }
+        }
+      }
+    }
+    const bool last = nt != t || nn0 != n0;
+    const bool have = nt < t1;
+    int nks = 0;
+    if (have && np < npe) {
+      if (np != p || nt != t) pr.get(pairs, np);
+      chain_load<T, RB>(nxt, pr, nn0, nk0, src, Bb, lr, lk);
+      nks = min(CHAIN_KS, (pr.K - nk0 + 3) / 4);
+    }
+#pragma unroll
+    for (int s = 0; s < CHAIN_KS; ++s) {
+      if (s >= cks) break;  // uniform
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = Acc<T>::mfma(cur.a[s][i], cur.b[s][j], acc[i][j]);
+    }
+    if (last) {
+      const zmpo_task& tk = tasks[t];
+      if (!TO_LDS) {
+        T* __restrict__ cb = C + c_base + tk.c_off;
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (rok[i][r]) {
+              const int64_t ro = (int64_t)mb[i][r] * tk.c_smhi + (int64_t)mr[i][r] * tk.c_smlo;
+#pragma unroll
+              for (int j = 0; j < 2; ++j) {
+                const int n = n0 + j * 16 + lr;
+                if (n < tN) {
+                  T* q = cb + ro + (int64_t)n * tk.c_sn;
+                  const T v = acc[i][j][r];
+                  *q = beta ? *q + v : v;
+                }
+              }
+            }
+          }
+      } else {
+        const int co = tk.c_off;
+        const int w4 = (tN + 3) & ~3;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int n = n0 + j * 16 + lr;
+          if (n >= w4) continue;
+#pragma unroll
+          for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              dst[(co + n) * LD + i * 16 + Acc<T>::row(lk, r)] = acc[i][j][r];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < RB; ++i) acc[i][0] = acc[i][1] = accv{0, 0, 0, 0};
+    }
+    if (!have) break;
+    t = nt;
+    n0 = nn0;
+    p = np;
+    k0 = nk0;
+    tN = nN;
+    tpb = npb;
+    tpe = npe;
+#pragma unroll
+    for (int s_ = 0; s_ < CHAIN_KS; ++s_) {
+#pragma unroll
+      for (int i = 0; i < RB; ++i) cur.a[s_][i] = nxt.a[s_][i];
+      cur.b[s_][0] = nxt.b[s_][0];
+      cur.b[s_][1] = nxt.b[s_][1];
+    }
+    cks = nks;
+  }
+}
 
 template <typename T, int RB>
 __global__ __launch_bounds__(256) void chain_gemm_kernel(
     const zmpo_cunit* __restrict__ units, int64_t nunits, const zmpo_cgroup* __restrict__ groups,
-    const zmpo_task* __restrict__ tasks, const zmpo_pair* __restrict__ pairs,
-    const T* __restrict__ A, const T* __restrict__ B1, const T* __restrict__ B2,
-    T* __restrict__ C, int beta) {
-  typedef typename Acc<T>::type accv;
+    const zmpo_cblock* __restrict__ blocks, const zmpo_task* __restrict__ tasks,
+    const zmpo_pair* __restrict__ pairs, const T* __restrict__ A, const T* __restrict__ B1,
+    const T* __restrict__ B2, T* __restrict__ C, int beta, int kcols_max) {
   constexpr int TMv = 16 * RB, LD = TMv + 1;
-  T* __restrict__ mid = reinterpret_cast<T*>(chain_lds);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  T* __restrict__ atile = reinterpret_cast<T*>(chain_lds);
+  T* __restrict__ mid = atile + (int64_t)kcols_max * LD;
+  const int lane = threadIdx.x & 63, wv = wave_id();
   const int lr = lane & 15, lk = lane >> 4;
   const int64_t u = xcd_logical_block(blockIdx.x, gridDim.x);
   if (u >= nunits) return;  // workgroup-uniform
@@ -856,130 +1055,63 @@ __global__ __launch_bounds__(256) void chain_gemm_kernel(
   const zmpo_cgroup g = groups[un.group];
   const int mv = min(TMv, g.M - un.m0);
 
-  // ---- stage 1: intermediate tile into LDS
-  for (int t = g.t1_begin + wv; t < g.t1_end; t += 4) {
-    const zmpo_task tk = tasks[t];
-    for (int n0 = 0; n0 < tk.N; n0 += 32) {
-      accv acc[RB][2];
+  // ---- stage 0: A blocks -> atile.  Lane rows: r0 = lane % TM.
+  {
+    const int r0 = lane & (TMv - 1);
+    const bool ok0 = r0 < mv;
+    const int m = un.m0 + r0;
+    const int qb = ok0 ? m / g.rsplit : 0, qr = ok0 ? m % g.rsplit : 0;
+    for (int j = g.ab_begin + wv; j < g.ab_end; j += 4) {
+      const zmpo_cblock bk = blocks[j];
+      const T* __restrict__ a = A + un.a_base + bk.a_off +
+                                ((int64_t)qb * bk.a_smhi + (int64_t)qr * bk.a_smlo);
+      const int kp = (bk.K + 3) & ~3;
+      const int ne = TMv * kp;
+      for (int e0 = 0; e0 < ne; e0 += 64 * 8) {
+        T v[8];
 #pragma unroll
-      for (int i = 0; i < RB; ++i) acc[i][0] = acc[i][1] = accv{0, 0, 0, 0};
-      for (int p = tk.p_begin; p < tk.p_end; ++p) {
-        const zmpo_pair pr = pairs[p];
-        const T* __restrict__ a = A + un.a_base + pr.a_off;
-        const T* __restrict__ b = B1 + un.b1_base + pr.b_off;
-        const T sc = (T)pr.scale;
-        int64_t aro[RB];
-        bool aok[RB];
-#pragma unroll
-        for (int i = 0; i < RB; ++i) {
-          const int r = i * 16 + lr;
-          const int m = un.m0 + r;
-          aok[i] = r < mv;
-          aro[i] = aok[i] ? (int64_t)(m / pr.a_mlo) * pr.a_smhi + (int64_t)(m % pr.a_mlo) * pr.a_smlo : 0;
+        for (int i = 0; i < 8; ++i) {
+          const int k = (e0 + i * 64 + lane) / TMv;
+          v[i] = (ok0 && k < bk.K) ? a[(int64_t)k * bk.a_sk] : (T)0;
         }
-        int64_t bco[2];
-        bool bok[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int n = n0 + j * 16 + lr;
-          bok[j] = n < tk.N;
-          bco[j] = bok[j] ? (int64_t)(n / pr.b_nlo) * pr.b_snhi + (int64_t)(n % pr.b_nlo) * pr.b_snlo : 0;
-        }
-#pragma unroll 4
-        for (int k0 = 0; k0 < pr.K; k0 += 4) {
-          const int k = k0 + lk;
-          const bool kok = k < pr.K;
-          T av[RB], bv[2];
-#pragma unroll
-          for (int i = 0; i < RB; ++i)
-            av[i] = (aok[i] && kok) ? a[aro[i] + (int64_t)k * pr.a_sk] * sc : (T)0;
-#pragma unroll
-          for (int j = 0; j < 2; ++j) bv[j] = (bok[j] && kok) ? b[(int64_t)k * pr.b_sk + bco[j]] : (T)0;
-#pragma unroll
-          for (int i = 0; i < RB; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = Acc<T>::mfma(av[i], bv[j], acc[i][j]);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int n = n0 + j * 16 + lr;
-        if (n < tk.N) {
-#pragma unroll
-          for (int i = 0; i < RB; ++i)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-              mid[(int64_t)(tk.c_off + n) * LD + i * 16 + Acc<T>::row(lk, r)] = acc[i][j][r];
+        for (int i = 0; i < 8; ++i) {
+          const int e = e0 + i * 64 + lane;
+          if (e < ne) atile[(bk.col + e / TMv) * LD + r0] = v[i];
         }
       }
     }
   }
+  // this lane's output rows (stage 2), split once
+  int mb[RB][4], mr[RB][4];
+  bool rok[RB][4];
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = i * 16 + Acc<T>::row(lk, r);
+      rok[i][r] = row < mv;
+      const int m = un.m0 + row;
+      mb[i][r] = rok[i][r] ? m / g.rsplit : 0;
+      mr[i][r] = rok[i][r] ? m % g.rsplit : 0;
+    }
   __syncthreads();
-
-  // ---- stage 2: output blocks from the LDS intermediate
-  for (int t = g.t2_begin + wv; t < g.t2_end; t += 4) {
-    const zmpo_task tk = tasks[t];
-    for (int n0 = 0; n0 < tk.N; n0 += 32) {
-      accv acc[RB][2];
-#pragma unroll
-      for (int i = 0; i < RB; ++i) acc[i][0] = acc[i][1] = accv{0, 0, 0, 0};
-      for (int p = tk.p_begin; p < tk.p_end; ++p) {
-        const zmpo_pair pr = pairs[p];
-        const T* __restrict__ b = B2 + un.b2_base + pr.b_off;
-        const T sc = (T)pr.scale;
-        const int c2 = (int)pr.a_off;
-        int64_t bco[2];
-        bool bok[2];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int n = n0 + j * 16 + lr;
-          bok[j] = n < tk.N;
-          bco[j] = bok[j] ? (int64_t)(n / pr.b_nlo) * pr.b_snhi + (int64_t)(n % pr.b_nlo) * pr.b_snlo : 0;
-        }
-#pragma unroll 4
-        for (int k0 = 0; k0 < pr.K; k0 += 4) {
-          const int k = k0 + lk;
-          const bool kok = k < pr.K;
-          T av[RB], bv[2];
-#pragma unroll
-          for (int i = 0; i < RB; ++i) av[i] = kok ? mid[(c2 + k) * LD + i * 16 + lr] * sc : (T)0;
-#pragma unroll
-          for (int j = 0; j < 2; ++j) bv[j] = (bok[j] && kok) ? b[(int64_t)k * pr.b_sk + bco[j]] : (T)0;
-#pragma unroll
-          for (int i = 0; i < RB; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = Acc<T>::mfma(av[i], bv[j], acc[i][j]);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int n = n0 + j * 16 + lr;
-        if (n < tk.N) {
-#pragma unroll
-          for (int i = 0; i < RB; ++i)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int row = i * 16 + Acc<T>::row(lk, r);
-              if (row < mv) {
-                const int m = un.m0 + row;
-                const int64_t off = un.c_base + tk.c_off + (int64_t)(m / tk.c_mlo) * tk.c_smhi +
-                                    (int64_t)(m % tk.c_mlo) * tk.c_smlo + (int64_t)n * tk.c_sn;
-                const T v = acc[i][j][r];
-                C[off] = beta ? C[off] + v : v;
-              }
-            }
-        }
-      }
-    }
-  }
+  // ---- stage 1: intermediate columns into LDS
+  chain_stage<T, RB, true>(g.t1_begin, g.t1_end, tasks, pairs, atile, B1 + un.b1_base, mid,
+                           nullptr, 0, mb, mr, rok, 0, wv, lr, lk);
+  __syncthreads();
+  // ---- stage 2: output blocks
+  chain_stage<T, RB, false>(g.t2_begin, g.t2_end, tasks, pairs, mid, B2 + un.b2_base, nullptr,
+                            C, un.c_base, mb, mr, rok, beta, wv, lr, lk);
 }
 
 template <typename T, int RB>
-int launch_chain(int ncols_max, const zmpo_cunit* units, int64_t nunits, const zmpo_cgroup* groups,
-                 const zmpo_task* tasks, const zmpo_pair* pairs, const void* A, const void* B1,
-                 const void* B2, void* C, int beta, hipStream_t s) {
-  const size_t lds = (size_t)ncols_max * (16 * RB + 1) * sizeof(T);
-  if (lds > 160 * 1024) return set_error(ZMPO_ERR_ARG, "chaingemm: intermediate exceeds 160 KiB of LDS");
+int launch_chain(int kcols_max, int ncols_max, const zmpo_cunit* units, int64_t nunits,
+                 const zmpo_cgroup* groups, const zmpo_cblock* blocks, const zmpo_task* tasks,
+                 const zmpo_pair* pairs, const void* A, const void* B1, const void* B2, void* C,
+                 int beta, hipStream_t s) {
+  const size_t lds = (size_t)(kcols_max + ncols_max) * (16 * RB + 1) * sizeof(T);
+  if (lds > 160 * 1024) return set_error(ZMPO_ERR_ARG, "chaingemm: row tile exceeds 160 KiB of LDS");
   if (lds > 64 * 1024) {
     int rc = check_hip(hipFuncSetAttribute((const void*)chain_gemm_kernel<T, RB>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
@@ -987,8 +1119,8 @@ int launch_chain(int ncols_max, const zmpo_cunit* units, int64_t nunits, const z
     if (rc) return rc;
   }
   hipLaunchKernelGGL((chain_gemm_kernel<T, RB>), dim3((unsigned)nunits), dim3(256), lds, s, units,
-                     nunits, groups, tasks, pairs, (const T*)A, (const T*)B1, (const T*)B2, (T*)C,
-                     beta);
+                     nunits, groups, blocks, tasks, pairs, (const T*)A, (const T*)B1,
+                     (const T*)B2, (T*)C, beta, kcols_max);
   return check_hip(hipGetLastError(), "chain_gemm launch");
 }
 
@@ -1232,22 +1364,23 @@ int zmpo_shapedgemm(int dtype, int shape, const zmpo_task* tasks, const zmpo_uni
   return set_error(ZMPO_ERR_ARG, "shapedgemm: unsupported dtype");
 }
 
-int zmpo_chaingemm(int dtype, int rb, int ncols_max, const zmpo_cunit* units, int64_t nunits,
-                   const zmpo_cgroup* groups, const zmpo_task* tasks, const zmpo_pair* pairs,
-                   const void* A, const void* B1, const void* B2, void* C, int beta,
-                   void* stream) {
+int zmpo_chaingemm(int dtype, int rb, int kcols_max, int ncols_max, const zmpo_cunit* units,
+                   int64_t nunits, const zmpo_cgroup* groups, const zmpo_cblock* blocks,
+                   const zmpo_task* tasks, const zmpo_pair* pairs, const void* A, const void* B1,
+                   const void* B2, void* C, int beta, void* stream) {
   if (nunits <= 0) return ZMPO_OK;
-  if (!units || !groups || !tasks || !pairs || !A || !B1 || !B2 || !C)
+  if (!units || !groups || !blocks || !tasks || !pairs || !A || !B1 || !B2 || !C)
     return set_error(ZMPO_ERR_ARG, "chaingemm: null pointer");
   if (nunits > 0x7fffffffLL) return set_error(ZMPO_ERR_ARG, "chaingemm: too many units");
-  if (ncols_max < 1) return set_error(ZMPO_ERR_ARG, "chaingemm: ncols_max must be >= 1");
+  if (ncols_max < 1 || kcols_max < 1)
+    return set_error(ZMPO_ERR_ARG, "chaingemm: kcols_max and ncols_max must be >= 1");
   hipStream_t s = (hipStream_t)stream;
   if (dtype == ZMPO_F64) {
-    if (rb == 1) return launch_chain<double, 1>(ncols_max, units, nunits, groups, tasks, pairs, A, B1, B2, C, beta, s);
-    if (rb == 2) return launch_chain<double, 2>(ncols_max, units, nunits, groups, tasks, pairs, A, B1, B2, C, beta, s);
+    if (rb == 1) return launch_chain<double, 1>(kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
+    if (rb == 2) return launch_chain<double, 2>(kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
   } else if (dtype == ZMPO_F32) {
-    if (rb == 1) return launch_chain<float, 1>(ncols_max, units, nunits, groups, tasks, pairs, A, B1, B2, C, beta, s);
-    if (rb == 2) return launch_chain<float, 2>(ncols_max, units, nunits, groups, tasks, pairs, A, B1, B2, C, beta, s);
+    if (rb == 1) return launch_chain<float, 1>(kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
+    if (rb == 2) return launch_chain<float, 2>(kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
   }
   return set_error(ZMPO_ERR_ARG, "chaingemm: unsupported dtype or row-block count");
 }

@@ -34,6 +34,7 @@ TOP3_PERM = (0, 3, 1, 2, 4)
 DEFAULT_BUDGET = 32 * 2 ** 30     # bytes of intermediates per op chunk
 # sigma stages 2-3 as one chained launch (intermediate top2 kept in LDS)
 FUSED = os.environ.get('ZMPO_FUSED', '1') != '0'
+CHAIN_RB = int(os.environ.get('ZMPO_CHAIN_RB', '1'))   # row tile of the chained launch: 16*rb
 
 
 def case(porb, isite, ncsite=1):
@@ -196,12 +197,18 @@ class SigmaProgram:
         self.inter = [0, 0, 0]
         structs = {}
         if self.fused:
+            _, poff_i, pmap_i = mpo.padded(isite)
+            _, poff_j, pmap_j = mpo.padded(jsite)
+            rep = {}
+            for p in range(nops):
+                rep.setdefault(case(mpo.opers[p], isite, 2), p)
             for c, pl in plans.items():
                 t2, t3 = pl[5], pl[6]
                 nr = 1 << 24
                 mid_key = t2.blk_isyms[:, 0] * nr + t2.blk_isyms[:, 2]
                 out_key = t3.blk_isyms[:, 0] * nr + t3.blk_isyms[:, 1]
-                structs[c] = chain_structure(pl[1], pl[2], mid_key, out_key, t2.blk_off)
+                structs[c] = chain_structure(pl[1], pl[2], mid_key, out_key, t2.blk_off,
+                                             pmap_i[rep[c]], pmap_j[rep[c]])
         for ch in self.chunks:
             offs = [0, 0, 0]
             parts = [[], [], [], []]
@@ -219,7 +226,8 @@ class SigmaProgram:
                 parts[1].append(_instantiate(pl[1], offs[0], wo, offs[1], 1.0, shift[1]))
                 parts[2].append(_instantiate(pl[2], offs[1], vo, offs[2], 1.0, shift[2]))
                 parts[3].append(_instantiate(pl[3], offs[2], ro, 0, float(hpwts[p]), 0))
-                inst.append((self.case_of[p], offs[0], wo, vo, offs[2]))
+                if self.fused:
+                    inst.append((self.case_of[p], offs[0], poff_i[p], poff_j[p], offs[2]))
                 for k in range(3):
                     offs[k] += sizes[k]
             progs = []
@@ -232,7 +240,7 @@ class SigmaProgram:
             for k in range(4):
                 self.flops_stage[k] += progs[k].flops
             self.stages.append(progs)
-            self.chains.append(ChainProgram(structs, inst) if self.fused else None)
+            self.chains.append(ChainProgram(structs, inst, rb=CHAIN_RB) if self.fused else None)
             self.inter = [max(a, b) for a, b in zip(self.inter, offs)]
         self.flops = int(self.flops_stage.sum())
         self._buffers = {}
@@ -264,7 +272,8 @@ class SigmaProgram:
         for st, chain in zip(self.stages, self.chains):
             run_program(st[0], L, psi, t1, dtype)
             if fused:
-                run_chain(chain, t1, Wi, Wj, t3, dtype)
+                run_chain(chain, t1, mpo.padded_buffer(self.isite, dtype),
+                          mpo.padded_buffer(self.isite + 1, dtype), t3, dtype)
             else:
                 run_program(st[1], t1, Wi, t2, dtype)
                 run_program(st[2], t2, Wj, t3, dtype)

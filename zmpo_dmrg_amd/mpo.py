@@ -182,6 +182,48 @@ class MPOStore:
         self.buf = [dv.upload(h, 0) for h in self.host]
         return self
 
+    def padded(self, i):
+        """Site i re-laid for the chained sigma launch: every block (w_l, w_r,
+        1, 1) of every op as a zero-padded row-major [ceil4(w_l)][ceil32(w_r)]
+        matrix, so that MFMA operand loads need no bounds tests.  Returns
+        (host values, op offsets, per-op {block offset: (padded offset, row
+        stride)}); built once per site."""
+        if not hasattr(self, '_pad'):
+            self._pad = {}
+        if i not in self._pad:
+            parts, offs, maps = [], [], []
+            tot = 0
+            for p, q in enumerate(self.sites[i]):
+                d = q.blk_dims
+                if len(d) and (np.any(d[:, 2] != 1) or np.any(d[:, 3] != 1)):
+                    raise NotImplementedError('padded MPO layout needs 1x1 physical blocks')
+                K = d[:, 0].astype(np.int64) if len(d) else np.zeros(0, np.int64)
+                N = d[:, 1].astype(np.int64) if len(d) else np.zeros(0, np.int64)
+                Kp, Np = (K + 3) // 4 * 4, (N + 31) // 32 * 32
+                sz = Kp * Np
+                new_off = np.concatenate([[0], np.cumsum(sz)[:-1]]).astype(np.int64) if len(sz) else sz
+                out = np.zeros(int(sz.sum()))
+                v = self.host[i][self.off[i][p]:self.off[i][p] + q.size_allowed]
+                for j in range(len(sz)):
+                    blk = v[q.blk_off[j]:q.blk_off[j] + K[j] * N[j]].reshape(K[j], N[j])
+                    out[new_off[j]:new_off[j] + sz[j]].reshape(Kp[j], Np[j])[:K[j], :N[j]] = blk
+                maps.append({int(o): (int(n), int(w)) for o, n, w in zip(q.blk_off, new_off, Np)})
+                offs.append(tot)
+                tot += len(out)
+                parts.append(out)
+            host = np.concatenate(parts) if parts else np.zeros(1)
+            self._pad[i] = {'host': host, 'off': np.array(offs, np.int64), 'map': maps, 'dev': {}}
+        e = self._pad[i]
+        return e['host'], e['off'], e['map']
+
+    def padded_buffer(self, i, dtype=0):
+        """Device copy of padded(i)[0] in the given precision."""
+        self.padded(i)
+        e = self._pad[i]
+        if dtype not in e['dev']:
+            e['dev'][dtype] = dv.upload(e['host'], dtype)
+        return e['dev'][dtype]
+
     def buffer(self, i, dtype=0):
         if dtype == 0:
             if self.buf[i] is None:

@@ -708,16 +708,20 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None)
                                      dv.ptr(work), beta, st))
 
 
-def chain_structure(first, second, mid_key, out_key, mid_off):
+def chain_structure(first, second, mid_key, out_key, mid_off, b1_map, b2_map):
     """Row-group structure of two chained plans (one operator case).
 
     ``first`` lowers X = A B1 (its tasks are the intermediate blocks, in
     block order, with c_off = mid_off), ``second`` lowers C = X B2 (pair
     a_off = an intermediate block offset).  mid_key / out_key give the row
     group of every intermediate / output block; the rows of blocks of one
-    group coincide.  Returns (groups, tasks, pairs) for zmpo_chaingemm with
-    offsets relative to each operand's base; groups without output blocks
-    are dropped (their intermediate is never read).
+    group coincide.  b1_map / b2_map send every B block offset to its
+    zero-padded copy (offset, row stride): the padded B blocks are
+    [ceil4(K)][ceil32(N)] row-major.  A-tile and intermediate columns are
+    reserved in multiples of 4 (the padding columns hold zeros).  Returns
+    (groups, tasks, pairs, blocks) for zmpo_chaingemm with offsets relative
+    to each operand's base; groups without output blocks are dropped (their
+    intermediate is never read).
     """
     t1, p1 = first['tasks'], first['pairs']
     t2, p2 = second['tasks'], second['pairs']
@@ -733,7 +737,7 @@ def chain_structure(first, second, mid_key, out_key, mid_off):
     order1 = np.lexsort((np.arange(len(t1)), grp_of_mid))
     order1 = order1[grp_of_mid[order1] >= 0]
     g1 = grp_of_mid[order1]
-    w1 = t1['N'][order1].astype(np.int64)
+    w1 = (t1['N'][order1].astype(np.int64) + 3) // 4 * 4
     csum = np.cumsum(w1) - w1
     start = np.concatenate([[0], np.flatnonzero(np.diff(g1)) + 1]) if len(g1) else np.zeros(0, np.int64)
     first_of = np.repeat(csum[start], np.diff(np.append(start, len(g1)))) if len(g1) else csum
@@ -768,9 +772,25 @@ def chain_structure(first, second, mid_key, out_key, mid_off):
     src[pos1] = order1
     src[pos2] = order2
     stage[pos2] = 1
+    p1 = p1.copy()
     p2 = p2.copy()
     if len(pa):
         p2['a_off'] = col[pa]
+    for pp, bmap in ((p1, b1_map), (p2, b2_map)):
+        if len(pp) == 0:
+            continue
+        if np.any(pp['b_nlo'] < np.repeat(t1['N'] if pp is p1 else t2['N'],
+                                          ((t1 if pp is p1 else t2)['p_end'] -
+                                           (t1 if pp is p1 else t2)['p_begin']).astype(np.int64))):
+            raise NotImplementedError('chained plans: B column map is not single-level')
+        nb = np.array([bmap[int(o)] for o in pp['b_off']], dtype=np.int64).reshape(-1, 2)
+        pp['b_off'] = nb[:, 0]
+        pp['b_sk'] = nb[:, 1]
+        pp['b_snlo'] = 1
+        pp['b_snhi'] = 0
+        pp['b_nlo'] = 1 << 30
+    if np.any(p1['scale'] != 1.0) or np.any(p2['scale'] != 1.0):
+        raise NotImplementedError('chained plans: pair scales must be 1')
     for ti in range(len(tasks)):
         j = int(src[ti])
         if stage[ti] == 0:
@@ -787,6 +807,26 @@ def chain_structure(first, second, mid_key, out_key, mid_off):
         tasks['p_end'][ti] = pcount
         pairs_parts.append(pp)
     pairs = np.concatenate(pairs_parts) if pairs_parts else np.zeros(0, _lib.PAIR_DTYPE)
+    # distinct first-stage A blocks per group, staged as A-tile columns
+    blocks = []
+    kcols = np.zeros(len(keys), np.int64)
+    ab = np.zeros(len(keys) + 1, np.int64)
+    for gi in range(len(keys)):
+        ab[gi] = len(blocks)
+        lo = tasks['p_begin'][tb1[gi]] if n1[gi] else 0
+        hi = tasks['p_end'][tb1[gi] + n1[gi] - 1] if n1[gi] else 0
+        seen = {}
+        for q in range(lo, hi):
+            ao = int(pairs['a_off'][q])
+            if ao not in seen:
+                seen[ao] = int(kcols[gi])
+                blocks.append((ao, int(pairs['a_smhi'][q]), int(pairs['a_smlo'][q]),
+                               int(pairs['a_sk'][q]), int(pairs['K'][q]), int(kcols[gi]),
+                               int(pairs['a_mlo'][q]), 0))
+                kcols[gi] += (int(pairs['K'][q]) + 3) // 4 * 4
+            pairs['a_off'][q] = seen[ao]
+    ab[len(keys)] = len(blocks)
+    blocks = np.array(blocks, dtype=_lib.CBLOCK_DTYPE) if blocks else np.zeros(0, _lib.CBLOCK_DTYPE)
     groups = np.zeros(len(keys), _lib.CGROUP_DTYPE)
     Mg = np.zeros(len(keys), np.int64)
     Mg[g2] = t2['M'][order2]
@@ -798,29 +838,55 @@ def chain_structure(first, second, mid_key, out_key, mid_off):
     groups['t1_end'] = tb1 + n1
     groups['t2_begin'] = tb1 + n1
     groups['t2_end'] = tb1 + n1 + n2
-    return groups, tasks, pairs
+    groups['kcols'] = kcols
+    groups['ab_begin'] = ab[:-1]
+    groups['ab_end'] = ab[1:]
+    # every block and output task of a group splits its rows the same way
+    rsplit = np.zeros(len(keys), np.int64)
+    for gi in range(len(keys)):
+        ms = set(blocks['a_mlo'][ab[gi]:ab[gi + 1]].tolist()) if len(blocks) else set()
+        ms |= set(tasks['c_mlo'][groups['t2_begin'][gi]:groups['t2_end'][gi]].tolist())
+        if len(ms) > 1:
+            raise NotImplementedError('chained plans: row maps differ inside a row group')
+        rsplit[gi] = ms.pop() if ms else 1
+    groups['rsplit'] = rsplit
+    if len(blocks):
+        r = np.maximum(Mg[np.repeat(np.arange(len(keys)), np.diff(ab))], 1) - 1
+        big = (r // blocks['a_mlo']) * blocks['a_smhi'] + np.minimum(r, blocks['a_mlo'] - 1) * \
+            blocks['a_smlo'] + (blocks['K'].astype(np.int64) - 1) * blocks['a_sk']
+        if big.max() >= 2 ** 31:
+            raise NotImplementedError('chained plans: an A block exceeds 2^31 elements')
+    return groups, tasks, pairs, blocks
+
+
+CHAIN_CLASSES = (96, 160, 256)   # LDS size classes (A-tile + intermediate columns)
 
 
 class ChainProgram:
-    """One zmpo_chaingemm launch: the row-group structures of several cases
-    instantiated for many operators (units = row tiles of 16*rb rows)."""
+    """zmpo_chaingemm launches: the row-group structures of several cases
+    instantiated for many operators (units = row tiles of 16*rb rows).
+    Units are split by the LDS their group needs (kcols + ncols, classes
+    CHAIN_CLASSES), one launch per class, so small groups run at full
+    occupancy; every output element is still written by exactly one unit."""
 
     def __init__(self, structures, instances, rb=1):
-        """structures: {case: (groups, tasks, pairs)}; instances: list of
-        (case, a_base, b1_base, b2_base, c_base)."""
+        """structures: {case: (groups, tasks, pairs, blocks)}; instances: list
+        of (case, a_base, b1_base, b2_base, c_base)."""
         self.rb = rb
         self._structs, self._inst = structures, instances
         tm = 16 * rb
-        gparts, tparts, pparts = [], [], []
-        gbase, tbase, pbase = {}, 0, 0
+        gparts, tparts, pparts, bparts = [], [], [], []
+        gbase, tbase, pbase, bbase = {}, 0, 0, 0
         self._gspan = {}
         ng = 0
         for c in sorted(structures):
-            g, t, p = structures[c]
+            g, t, p, b = structures[c]
             g = g.copy()
             t = t.copy()
             for f in ('t1_begin', 't1_end', 't2_begin', 't2_end'):
                 g[f] += tbase
+            for f in ('ab_begin', 'ab_end'):
+                g[f] += bbase
             t['p_begin'] += pbase
             t['p_end'] += pbase
             gbase[c] = ng
@@ -828,23 +894,24 @@ class ChainProgram:
             ng += len(g)
             tbase += len(t)
             pbase += len(p)
+            bbase += len(b)
             gparts.append(g)
             tparts.append(t)
             pparts.append(p)
+            bparts.append(b)
         self.groups_host = np.concatenate(gparts) if gparts else np.zeros(0, _lib.CGROUP_DTYPE)
         self.tasks_host = np.concatenate(tparts) if tparts else np.zeros(0, _lib.TASK_DTYPE)
         self.pairs_host = np.concatenate(pparts) if pparts else np.zeros(0, _lib.PAIR_DTYPE)
-        self.ncols_max = int(max(1, self.groups_host['ncols'].max())) if ng else 1
+        self.blocks_host = np.concatenate(bparts) if bparts else np.zeros(0, _lib.CBLOCK_DTYPE)
         uparts = []
         self.flops = 0
         t, p = self.tasks_host, self.pairs_host
         prow = np.repeat(np.arange(len(t)), (t['p_end'] - t['p_begin']).astype(np.int64))
         pfl = 2 * t['M'][prow].astype(np.int64) * t['N'][prow] * p['K'].astype(np.int64)
         tfl = np.bincount(prow, weights=pfl, minlength=len(t)) if len(p) else np.zeros(len(t))
-        self._tfl = tfl
         self.instance_cases = [c for c, *_ in instances]
         for c, a0, b10, b20, c0 in instances:
-            g, tt, _ = structures[c]
+            g = structures[c][0]
             if len(g) == 0:
                 continue
             ntile = np.maximum((g['M'].astype(np.int64) + tm - 1) // tm, 1)
@@ -856,65 +923,79 @@ class ChainProgram:
             u['m0'] = m0
             uparts.append(u)
             gl = gbase[c]
-            self.flops += int(sum(tfl[self.groups_host['t1_begin'][gl]:self.groups_host['t2_end'][gl + len(g) - 1]]))
+            self.flops += int(tfl[self.groups_host['t1_begin'][gl]:
+                                  self.groups_host['t2_end'][gl + len(g) - 1]].sum())
         self.units_host = np.concatenate(uparts) if uparts else np.zeros(0, _lib.CUNIT_DTYPE)
         self.nunits = len(self.units_host)
+        # LDS classes
+        g = self.groups_host
+        width = (g['kcols'] + g['ncols']).astype(np.int64)
+        cls = np.searchsorted(np.asarray(CHAIN_CLASSES), width[self.units_host['group']]) \
+            if self.nunits else np.zeros(0, np.int64)
+        self.launches = []
+        for k in range(len(CHAIN_CLASSES) + 1):
+            sel = np.flatnonzero(cls == k)
+            if len(sel) == 0:
+                continue
+            us = self.units_host[sel]
+            gsel = us['group']
+            self.launches.append({'units': us, 'kmax': int(max(1, g['kcols'][gsel].max())),
+                                  'nmax': int(max(1, g['ncols'][gsel].max()))})
         self._dev = None
 
     def with_rb(self, rb):
-        """The same launch with 16*rb-row tiles."""
+        """The same launches with 16*rb-row tiles."""
         return self if rb == self.rb else ChainProgram(self._structs, self._inst, rb)
 
     def alg_bytes(self, elsize=8):
-        """Algorithmic HBM bytes of one launch: the first-stage A blocks and
-        the output blocks moved once, and every operator's B1/B2 blocks read
-        once; the intermediate stays on chip."""
+        """Algorithmic HBM bytes: the first-stage A blocks and the output
+        blocks moved once, and every operator's B1/B2 blocks read once; the
+        intermediate stays on chip."""
         if self.nunits == 0:
             return 0
         t, p, g = self.tasks_host, self.pairs_host, self.groups_host
-        cols = np.zeros(len(g), np.int64)
+        cols = g['kcols'].astype(np.int64).copy()
         for gi, gg in enumerate(g):
-            if gg['t1_end'] > gg['t1_begin']:
-                pp = p[t['p_begin'][gg['t1_begin']]:t['p_end'][gg['t1_end'] - 1]]
-                _, ia = np.unique(pp['a_off'], return_index=True)
-                cols[gi] += int(pp['K'][ia].sum())
             cols[gi] += int(t['N'][gg['t2_begin']:gg['t2_end']].sum())
         u = self.units_host
         rows = np.minimum(16 * self.rb, g['M'][u['group']].astype(np.int64) - u['m0'])
         tot = int(np.sum(rows * cols[u['group']]))
         prow = np.repeat(np.arange(len(t)), (t['p_end'] - t['p_begin']).astype(np.int64))
         bsz = t['N'][prow].astype(np.int64) * p['K']
+        stage2 = np.zeros(len(p), np.int64)
+        for gg in g:
+            if gg['t2_end'] > gg['t2_begin']:
+                stage2[t['p_begin'][gg['t2_begin']]:t['p_end'][gg['t2_end'] - 1]] = 1
         for c in self.instance_cases:
             gl, gn = self._gspan[c]
             if gn == 0:
                 continue
             lo = t['p_begin'][g['t1_begin'][gl]]
             hi = t['p_end'][g['t2_end'][gl + gn - 1] - 1]
-            stg = np.zeros(hi - lo, np.int64)
-            for gg in g[gl:gl + gn]:
-                stg[t['p_begin'][gg['t2_begin']] - lo:t['p_end'][gg['t2_end'] - 1] - lo] = 1 \
-                    if gg['t2_end'] > gg['t2_begin'] else 0
-            key = p['b_off'][lo:hi] * 2 + stg
+            key = p['b_off'][lo:hi] * 2 + stage2[lo:hi]
             _, ib = np.unique(key, return_index=True)
             tot += int(bsz[lo:hi][ib].sum())
         return elsize * tot
 
     def device(self):
         if self._dev is None:
-            self._dev = tuple(dv.upload_bytes(x) for x in (self.units_host, self.groups_host,
-                                                           self.tasks_host, self.pairs_host))
+            common = tuple(dv.upload_bytes(x) for x in (self.groups_host, self.blocks_host,
+                                                        self.tasks_host, self.pairs_host))
+            self._dev = (common, [dv.upload_bytes(l_['units']) for l_ in self.launches])
         return self._dev
 
 
 def run_chain(prog, A, B1, B2, C, dtype=_lib.F64, beta=0, stream=None):
-    """Launch one zmpo_chaingemm: C (=|+=) (A B1) B2 over the program's row tiles."""
+    """Launch zmpo_chaingemm (one launch per LDS class): C (=|+=) (A B1) B2."""
     if prog.nunits == 0:
         return
-    u, g, t, p = prog.device()
+    (g, b, t, p), units = prog.device()
     st = dv.stream() if stream is None else stream
-    _lib.check(_lib.lib().zmpo_chaingemm(dtype, prog.rb, prog.ncols_max, dv.ptr(u), prog.nunits,
-                                         dv.ptr(g), dv.ptr(t), dv.ptr(p), dv.ptr(A), dv.ptr(B1),
-                                         dv.ptr(B2), dv.ptr(C), beta, st))
+    L = _lib.lib()
+    for l_, u in zip(prog.launches, units):
+        _lib.check(L.zmpo_chaingemm(dtype, prog.rb, l_['kmax'], l_['nmax'], dv.ptr(u),
+                                    len(l_['units']), dv.ptr(g), dv.ptr(b), dv.ptr(t), dv.ptr(p),
+                                    dv.ptr(A), dv.ptr(B1), dv.ptr(B2), dv.ptr(C), beta, st))
 
 
 def gather(dst, src, idx, dtype_dst=_lib.F64, dtype_src=_lib.F64):
