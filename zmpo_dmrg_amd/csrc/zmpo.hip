@@ -889,21 +889,25 @@ struct ChainPair {
 
 // Operands of the k-steps [k0, k0 + 4*CHAIN_KS) of one pair, 32-column tile
 // n0.  Only k-steps that start below K are loaded (uniform test); lanes past
-// K inside the last k-step read the zero padding of both operands.
+// K inside the last k-step read the zero padding of both operands.  Bases
+// are wave-uniform (scalar registers); each lane adds a fixed 32-bit offset.
 template <typename T, int RB>
 __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPair& pr, int n0,
                                            int k0, const T* __restrict__ src,
-                                           const T* __restrict__ Bb, int lr, int lk) {
+                                           const T* __restrict__ Bb, int lane_a, int lr, int lk) {
   constexpr int LD = 16 * RB + 1;
-  const T* __restrict__ b = Bb + pr.b_off + (int64_t)(k0 + lk) * pr.ldb + n0 + lr;
-  const T* __restrict__ a = src + (pr.c + k0 + lk) * LD + lr;
+  const int lane_b = lk * pr.ldb + lr;
+  const T* __restrict__ b0 = Bb + pr.b_off + (int64_t)k0 * pr.ldb + n0;
+  const T* __restrict__ a0 = src + (pr.c + k0) * LD;
 #pragma unroll
   for (int s = 0; s < CHAIN_KS; ++s) {
     if (k0 + 4 * s < pr.K) {
-      sl.b[s][0] = b[4 * s * pr.ldb];
-      sl.b[s][1] = b[4 * s * pr.ldb + 16];
+      const T* __restrict__ bs = b0 + 4 * s * pr.ldb;
+      sl.b[s][0] = bs[lane_b];
+      sl.b[s][1] = bs[lane_b + 16];
+      const T* __restrict__ as = a0 + 4 * s * LD;
 #pragma unroll
-      for (int i = 0; i < RB; ++i) sl.a[s][i] = a[4 * s * LD + i * 16];
+      for (int i = 0; i < RB; ++i) sl.a[s][i] = as[lane_a + i * 16];
     }
   }
 }
@@ -915,7 +919,116 @@ __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPai
 // (t0 + wave, step 4) as one flat sequence of items -- an item is a
 // 4*CHAIN_KS-deep k slice of one pair for one 32-column tile -- and loads
 // the operands of item i+1 (possibly of the next pair or task) while the
-// MFMAs of item i run.
+// MFMAs of item i run; the two operand slices alternate roles, so nothing
+// is copied between them.
+template <typename T, int RB, bool TO_LDS>
+struct ChainWalk {
+  typedef typename Acc<T>::type accv;
+  static constexpr int LD = 16 * RB + 1;
+  static constexpr int KD = 4 * CHAIN_KS;
+  const zmpo_task* __restrict__ tasks;
+  const zmpo_pair* __restrict__ pairs;
+  const T* __restrict__ src;
+  const T* __restrict__ Bb;
+  T* __restrict__ dst;
+  T* __restrict__ C;
+  int64_t c_base;
+  int t1, beta, lane_a, lr, lk;
+  // cursor of the item whose operands are being loaded
+  int t, n0, p, k0, tN, tpb, tpe;
+  ChainPair pr;
+  accv acc[RB][2];
+
+  // load the item under the cursor into sl; returns its k-step count
+  __device__ __forceinline__ int load(ChainSlice<T, RB>& sl) {
+    if (p >= tpe) return 0;
+    chain_load<T, RB>(sl, pr, n0, k0, src, Bb, lane_a, lr, lk);
+    return min(CHAIN_KS, (pr.K - k0 + 3) / 4);
+  }
+  // move the cursor to the next item; returns whether the previous item
+  // closed its (task, column tile), and sets (ct, cn0, cN) to that tile
+  __device__ __forceinline__ bool advance(int& ct, int& cn0, int& cN) {
+    ct = t;
+    cn0 = n0;
+    cN = tN;
+    if (p < tpe && k0 + KD < pr.K) {
+      k0 += KD;
+      return false;
+    }
+    k0 = 0;
+    ++p;
+    if (p < tpe) {
+      pr.get(pairs, p);
+      return false;
+    }
+    p = tpb;
+    n0 += 32;
+    if (n0 < tN) {
+      if (p < tpe) pr.get(pairs, p);
+      return true;
+    }
+    n0 = 0;
+    t += 4;
+    if (t < t1) {
+      tN = tasks[t].N;
+      tpb = tasks[t].p_begin;
+      tpe = tasks[t].p_end;
+      p = tpb;
+      if (p < tpe) pr.get(pairs, p);
+    }
+    return true;
+  }
+  __device__ __forceinline__ void mfma(const ChainSlice<T, RB>& sl, int ks) {
+#pragma unroll
+    for (int s = 0; s < CHAIN_KS; ++s) {
+      if (s >= ks) break;  // uniform
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = Acc<T>::mfma(sl.a[s][i], sl.b[s][j], acc[i][j]);
+    }
+  }
+  __device__ __forceinline__ void flush(int ct, int cn0, int cN, const int (&mb)[RB][4],
+                                        const int (&mr)[RB][4], const bool (&rok)[RB][4]) {
+    const zmpo_task& tk = tasks[ct];
+    if (!TO_LDS) {
+      T* __restrict__ cb = C + c_base + tk.c_off;
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (rok[i][r]) {
+            const int64_t ro = (int64_t)mb[i][r] * tk.c_smhi + (int64_t)mr[i][r] * tk.c_smlo;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              const int n = cn0 + j * 16 + lr;
+              if (n < cN) {
+                T* q = cb + ro + (int64_t)n * tk.c_sn;
+                const T v = acc[i][j][r];
+                *q = beta ? *q + v : v;
+              }
+            }
+          }
+        }
+    } else {
+      const int co = tk.c_off;
+      const int w4 = (cN + 3) & ~3;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int n = cn0 + j * 16 + lr;
+        if (n >= w4) continue;
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            dst[(co + n) * LD + i * 16 + Acc<T>::row(lk, r)] = acc[i][j][r];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < RB; ++i) acc[i][0] = acc[i][1] = accv{0, 0, 0, 0};
+  }
+};
+
 template <typename T, int RB, bool TO_LDS>
 __device__ __forceinline__ void chain_stage(
     int t0, int t1, const zmpo_task* __restrict__ tasks, const zmpo_pair* __restrict__ pairs,
@@ -923,118 +1036,47 @@ __device__ __forceinline__ void chain_stage(
     T* __restrict__ C, int64_t c_base, const int (&mb)[RB][4], const int (&mr)[RB][4],
     const bool (&rok)[RB][4], int beta, int wv, int lr, int lk) {
   typedef typename Acc<T>::type accv;
-  constexpr int LD = 16 * RB + 1;
-  constexpr int KD = 4 * CHAIN_KS;
-  int t = t0 + wv;
-  if (t >= t1) return;  // wave-uniform
-  int tN = tasks[t].N, tpb = tasks[t].p_begin, tpe = tasks[t].p_end;
-  int n0 = 0, p = tpb, k0 = 0;
-  ChainPair pr;
-  pr.K = 0;
-  ChainSlice<T, RB> cur, nxt;
-  int cks = 0;
-  if (p < tpe) {
-    pr.get(pairs, p);
-    chain_load<T, RB>(cur, pr, n0, k0, src, Bb, lr, lk);
-    cks = min(CHAIN_KS, (pr.K - k0 + 3) / 4);
-  }
-  accv acc[RB][2];
+  ChainWalk<T, RB, TO_LDS> w;
+  w.t = t0 + wv;
+  if (w.t >= t1) return;  // wave-uniform
+  w.tasks = tasks;
+  w.pairs = pairs;
+  w.src = src;
+  w.Bb = Bb;
+  w.dst = dst;
+  w.C = C;
+  w.c_base = c_base;
+  w.t1 = t1;
+  w.beta = beta;
+  w.lane_a = lk * (16 * RB + 1) + lr;
+  w.lr = lr;
+  w.lk = lk;
+  w.tN = tasks[w.t].N;
+  w.tpb = tasks[w.t].p_begin;
+  w.tpe = tasks[w.t].p_end;
+  w.n0 = 0;
+  w.k0 = 0;
+  w.p = w.tpb;
+  w.pr.K = 0;
+  if (w.p < w.tpe) w.pr.get(pairs, w.p);
 #pragma unroll
-  for (int i = 0; i < RB; ++i) acc[i][0] = acc[i][1] = accv{0, 0, 0, 0};
+  for (int i = 0; i < RB; ++i) w.acc[i][0] = w.acc[i][1] = accv{0, 0, 0, 0};
+  ChainSlice<T, RB> s0, s1;
+  int k_0 = w.load(s0), k_1 = 0;
+  int ct, cn0, cN;
   while (true) {
-    // advance the cursor (t, n0, p, k0) to the next item
-    int nt = t, nn0 = n0, np = p, nk0 = k0;
-    int nN = tN, npb = tpb, npe = tpe;
-    if (np < npe && nk0 + KD < pr.K) {
-      nk0 += KD;
-    } else {
-      nk0 = 0;
-      ++np;
-      if (np >= npe) {
-        np = npb;
-        nn0 += 32;
-        if (nn0 >= nN) {
-          nn0 = 0;
-          nt += 4;
-          if (nt < t1) {
-            nN = tasks[nt].N;
-            npb = tasks[nt].p_begin;
-            npe = tasks[nt].p_end;
-            np = npb;
-          }
-        }
-      }
-    }
-    const bool last = nt != t || nn0 != n0;
-    const bool have = nt < t1;
-    int nks = 0;
-    if (have && np < npe) {
-      if (np != p || nt != t) pr.get(pairs, np);
-      chain_load<T, RB>(nxt, pr, nn0, nk0, src, Bb, lr, lk);
-      nks = min(CHAIN_KS, (pr.K - nk0 + 3) / 4);
-    }
-#pragma unroll
-    for (int s = 0; s < CHAIN_KS; ++s) {
-      if (s >= cks) break;  // uniform
-#pragma unroll
-      for (int i = 0; i < RB; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = Acc<T>::mfma(cur.a[s][i], cur.b[s][j], acc[i][j]);
-    }
-    if (last) {
-      const zmpo_task& tk = tasks[t];
-      if (!TO_LDS) {
-        T* __restrict__ cb = C + c_base + tk.c_off;
-#pragma unroll
-        for (int i = 0; i < RB; ++i)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            if (rok[i][r]) {
-              const int64_t ro = (int64_t)mb[i][r] * tk.c_smhi + (int64_t)mr[i][r] * tk.c_smlo;
-#pragma unroll
-              for (int j = 0; j < 2; ++j) {
-                const int n = n0 + j * 16 + lr;
-                if (n < tN) {
-                  T* q = cb + ro + (int64_t)n * tk.c_sn;
-                  const T v = acc[i][j][r];
-                  *q = beta ? *q + v : v;
-                }
-              }
-            }
-          }
-      } else {
-        const int co = tk.c_off;
-        const int w4 = (tN + 3) & ~3;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int n = n0 + j * 16 + lr;
-          if (n >= w4) continue;
-#pragma unroll
-          for (int i = 0; i < RB; ++i)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-              dst[(co + n) * LD + i * 16 + Acc<T>::row(lk, r)] = acc[i][j][r];
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < RB; ++i) acc[i][0] = acc[i][1] = accv{0, 0, 0, 0};
-    }
+    bool last = w.advance(ct, cn0, cN);
+    bool have = w.t < t1;
+    if (have) k_1 = w.load(s1);
+    w.mfma(s0, k_0);
+    if (last) w.flush(ct, cn0, cN, mb, mr, rok);
     if (!have) break;
-    t = nt;
-    n0 = nn0;
-    p = np;
-    k0 = nk0;
-    tN = nN;
-    tpb = npb;
-    tpe = npe;
-#pragma unroll
-    for (int s_ = 0; s_ < CHAIN_KS; ++s_) {
-#pragma unroll
-      for (int i = 0; i < RB; ++i) cur.a[s_][i] = nxt.a[s_][i];
-      cur.b[s_][0] = nxt.b[s_][0];
-      cur.b[s_][1] = nxt.b[s_][1];
-    }
-    cks = nks;
+    last = w.advance(ct, cn0, cN);
+    have = w.t < t1;
+    if (have) k_0 = w.load(s0);
+    w.mfma(s1, k_1);
+    if (last) w.flush(ct, cn0, cN, mb, mr, rok);
+    if (!have) break;
   }
 }
 
