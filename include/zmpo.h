@@ -22,6 +22,13 @@
  *   zmpo_wavegemm        the same numeric loop, scheduled one wave per small
  *                        output tile (the block shapes of block-sparse
  *                        DMRG are mostly 5-150 wide).
+ *   zmpo_shapedgemm, zmpo_streamgemm, zmpo_skinnygemm
+ *                        the same numeric loop with LDS-staged block-shaped
+ *                        tiles (one tile per workgroup / unit ranges per
+ *                        workgroup / tall narrow blocks).
+ *   zmpo_chaingemm       the two middle stages of HVecQt (kernelQt.py:
+ *                        277-296, top1 x W[i] then x W[i+1]) as one launch
+ *                        with the intermediate top2 kept on chip.
  *   zmpo_dots, zmpo_lincomb, zmpo_precond
  *                        the vector algebra of the Davidson solver
  *                        (source/tools/pdvdson.py:131-283, 350-390), which the
@@ -159,6 +166,17 @@ int zmpo_streamgemm(int dtype, int shape, int kt, const zmpo_sunit* units, int64
                     int64_t ngroups, const zmpo_pair* pairs, const void* A, const void* B,
                     void* C, void* work, int beta, void* stream);
 
+/* Grouped GEMM for tall, narrow blocks (sigma stage L x psi of HVecQt,
+ * mpo_dmrg_kernelQt.py:270-276, with the same task/pair semantics as
+ * zmpo_blockgemm): one workgroup per zmpo_unit = rows [m0, m0 + 64*rb) x
+ * columns [n0, n0 + 64) of task `task` over pairs [p_begin, p_end) (slot
+ * unused: every unit owns its output).  Requirements (checked by the
+ * caller): every pair's A rows are single-level (a_mlo >= M) with a_sk == 1.
+ * rb in {1, 2}. */
+int zmpo_skinnygemm(int dtype, int rb, const zmpo_task* tasks, const zmpo_unit* units,
+                    int64_t nunits, const zmpo_pair* pairs, const void* A, const void* B, void* C,
+                    int beta, void* stream);
+
 /* Two chained grouped GEMMs over row groups, C = (A B1) B2 blockwise, for
  * the middle sigma stages of HVecQt (mpo_dmrg_kernelQt.py:277-296: top1 x
  * W[i] -> top2, top2 x W[i+1] -> top3).  All blocks of one row group share
@@ -195,12 +213,17 @@ typedef struct zmpo_cunit {
   int32_t group, m0;
 } zmpo_cunit;
 
-/* rb in {1, 2}; kcols_max / ncols_max >= every group's kcols / ncols (they
- * size the LDS: (kcols_max + ncols_max) * (16*rb + 1) elements <= 160 KiB). */
-int zmpo_chaingemm(int dtype, int rb, int kcols_max, int ncols_max, const zmpo_cunit* units,
-                   int64_t nunits, const zmpo_cgroup* groups, const zmpo_cblock* blocks,
-                   const zmpo_task* tasks, const zmpo_pair* pairs, const void* A, const void* B1,
-                   const void* B2, void* C, int beta, void* stream);
+/* rb in {1, 2}.  atile = 1 stages the A blocks in LDS (first-stage pair
+ * a_off = A-tile column); atile = 0 reads them from global memory through
+ * the block the first-stage pair names in its `pad` field (zmpo_cblock
+ * index), leaving the LDS to the intermediate.  kcols_max / ncols_max >=
+ * every group's kcols / ncols size the LDS: ((atile ? kcols_max : 0) +
+ * ncols_max) * (16*rb + 1) elements <= 160 KiB. */
+int zmpo_chaingemm(int dtype, int rb, int atile, int kcols_max, int ncols_max,
+                   const zmpo_cunit* units, int64_t nunits, const zmpo_cgroup* groups,
+                   const zmpo_cblock* blocks, const zmpo_task* tasks, const zmpo_pair* pairs,
+                   const void* A, const void* B1, const void* B2, void* C, int beta,
+                   void* stream);
 
 /* out[j] = sum_i X[j*ldx + i] * y[i] for j < nx (device out, float64;
  * work holds zmpo_dots_workspace(nx) doubles).  Fixed reduction order. */

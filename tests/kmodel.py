@@ -43,8 +43,9 @@ def emulate_wave(prog, A, B, C, dtype=0, beta=0, stream=None, kernel=None):
     if isinstance(prog, dict):
         prog = Program(prog['tasks'], prog['pairs'])
     from zmpo_dmrg_amd.qtensor import KERNEL, SHAPES
-    kernel = KERNEL if kernel is None else kernel
-    if kernel == 'tile':
+    if kernel is None:
+        kernel = getattr(prog, 'prefer', None) or KERNEL
+    if kernel in ('tile', 'skinny'):
         return emulate(prog, A, B, C, dtype, beta)
     A, B, C = _np(A), _np(B), _np(C)
     rb = prog.rb

@@ -24,7 +24,7 @@ def _dense_program(M, N, K, trans_a=False, trans_b=False):
 @pytest.mark.parametrize('shape', [(64, 64, 16), (37, 90, 33), (130, 7, 200), (1, 1, 1),
                                    (5, 9, 5000)])
 @pytest.mark.parametrize('ta,tb', [(False, False), (True, False), (False, True)])
-@pytest.mark.parametrize('kernel', ['wave', 'tile', 'shaped', 'stream'])
+@pytest.mark.parametrize('kernel', ['wave', 'tile', 'shaped', 'stream', 'skinny'])
 def test_blockgemm_dense_f64(gpu, shape, ta, tb, kernel):
     M, N, K = shape
     rng = np.random.default_rng(M * N + K)
@@ -86,7 +86,7 @@ def test_wavegemm_split_k_random_programs(gpu):
         assert np.abs(dv.download(C) - want).max() < 1e-10, (kernel, rb, shape, kt, upw)
 
 
-@pytest.mark.parametrize('kernel', ['wave', 'tile', 'shaped', 'stream'])
+@pytest.mark.parametrize('kernel', ['wave', 'tile', 'shaped', 'stream', 'skinny'])
 def test_blockgemm_beta_accumulates(gpu, kernel):
     M = N = K = 20
     rng = np.random.default_rng(2)
@@ -124,7 +124,7 @@ def test_blockgemm_real_plans_match_model(gpu, case4):
             A, B = rng.standard_normal(na), rng.standard_normal(nb)
             want = np.zeros(nc)
             emulate(pg, A, B, want)
-            for kernel, upw in (('shaped', None), ('stream', None), ('stream', 64)):
+            for kernel, upw in (('shaped', None), ('stream', None), ('stream', 64), ('skinny', None)):
                 pg.upw = upw
                 pg._sched = None
                 C = dv.zeros(nc, 0)
@@ -175,9 +175,10 @@ def test_errors_surface(gpu):
     assert lib.zmpo_last_error()
 
 
+@pytest.mark.parametrize('atile', [1, 0])
 @pytest.mark.parametrize('dtype', [0, 1])
 @pytest.mark.parametrize('rb', [1, 2])
-def test_chaingemm_real_plan_matches_two_stages(gpu, rb, dtype):
+def test_chaingemm_real_plan_matches_two_stages(gpu, rb, dtype, atile):
     """zmpo_chaingemm (sigma stages 2-3 with the intermediate in LDS) vs the
     CPU model of the two separate stages, on a real K=5 sigma plan."""
     from kmodel import host_model
@@ -192,6 +193,8 @@ def test_chaingemm_real_plan_matches_two_stages(gpu, rb, dtype):
     rng = np.random.default_rng(rb)
     for st, ch in zip(sp.stages, sp.chains):
         ch = ch.with_rb(rb)
+        for l_ in ch.launches:   # A staged in LDS (1) or read from global memory (0)
+            l_['atile'] = atile
         n1 = int(st[1].pairs_host['a_off'].max() + 5000)
         n2 = int(st[1].tasks_host['c_off'].max() + 5000)
         n3 = int(st[2].tasks_host['c_off'].max() + 5000)

@@ -848,6 +848,120 @@ int dispatch_stream(int shape, int kt, const zmpo_sunit* units, int64_t nunits, 
   return set_error(ZMPO_ERR_ARG, "streamgemm: kt must be 16 or 32");
 }
 
+// ------------------------------------------------------------- skinny GEMMs
+// Tall blocks with few columns (sigma stage 1: M = bra x MPO-bond rows in
+// the hundreds, N and K = sector dims of 5-60).  One workgroup = a strip of
+// 64*RB rows x up to 64 columns of one output block; each wave owns 16*RB
+// rows.  B (K x N, small) is staged once per 64-deep K chunk into LDS with
+// zero padding; A rows (single-level, K-contiguous) go straight from
+// global memory into the MFMA operand registers, all k-steps of a chunk
+// issued before their MFMAs, with the row clamped into the block and k
+// clamped to K-1 (those products meet zero B rows or are not stored), so
+// the inner loop has no bounds tests.  Tasks may have several pairs; each
+// output element is written once.
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void skinny_gemm_kernel(
+    const zmpo_task* __restrict__ tasks, const zmpo_unit* __restrict__ units, int64_t nunits,
+    const zmpo_pair* __restrict__ pairs, const T* __restrict__ A, const T* __restrict__ B,
+    T* __restrict__ C, int beta) {
+  typedef typename Acc<T>::type accv;
+  constexpr int KC = 64, NC = 64;
+  __shared__ T Bs[KC][NC + 1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = wave_id();
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t u = xcd_logical_block(blockIdx.x, gridDim.x);
+  if (u >= nunits) return;  // workgroup-uniform
+  const zmpo_unit un = units[u];
+  const zmpo_task& tk = tasks[un.task];
+  const int M = tk.M, N = tk.N;
+  const int nv = min(NC, N - un.n0);
+  const int nt = (nv + 15) >> 4;  // 16-column MFMA tiles in use (uniform)
+  const int rbase = un.m0 + wv * 16 * RB;
+  accv acc[RB][4];
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = accv{0, 0, 0, 0};
+  for (int p = un.p_begin; p < un.p_end; ++p) {
+    const zmpo_pair& pr = pairs[p];
+    const int K = pr.K;
+    // this lane's A rows (clamped into the block)
+    int64_t ro[RB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) ro[i] = (int64_t)min(rbase + i * 16 + lr, M - 1) * pr.a_smlo;
+    const T* __restrict__ a = A + pr.a_off;
+    const T sc = (T)pr.scale;
+    const bool bone = pr.b_nlo >= N;
+    for (int kc = 0; kc < K; kc += KC) {
+      const int kv = min(KC, K - kc);
+      __syncthreads();  // the previous chunk's readers are done with Bs
+      for (int e = tid; e < KC * NC; e += 256) {
+        const int k = e / NC, n = e % NC;
+        T v = (T)0;
+        if (k < kv && n < nv) {
+          const int nn = un.n0 + n;
+          const int64_t co = bone ? (int64_t)nn * pr.b_snlo
+                                  : (int64_t)(nn / pr.b_nlo) * pr.b_snhi + (int64_t)(nn % pr.b_nlo) * pr.b_snlo;
+          v = B[pr.b_off + (int64_t)(kc + k) * pr.b_sk + co] * sc;
+        }
+        Bs[k][n] = v;
+      }
+      __syncthreads();
+      T av[16][RB];
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        if (4 * s < kv) {
+          const int k = min(kc + 4 * s + lk, K - 1);
+#pragma unroll
+          for (int i = 0; i < RB; ++i) av[s][i] = a[ro[i] + k];
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        if (4 * s >= kv) break;  // uniform
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (j >= nt) break;  // uniform
+          const T bv = Bs[4 * s + lk][j * 16 + lr];
+#pragma unroll
+          for (int i = 0; i < RB; ++i) acc[i][j] = Acc<T>::mfma(av[s][i], bv, acc[i][j]);
+        }
+      }
+    }
+  }
+  // output: lane column n0 + j*16 + lr, rows rbase + i*16 + row(lk, r)
+  const bool cone = tk.c_mlo >= M;
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = rbase + i * 16 + Acc<T>::row(lk, r);
+      if (m >= M) continue;
+      const int64_t ro = tk.c_off + (cone ? (int64_t)m * tk.c_smlo
+                                          : (int64_t)(m / tk.c_mlo) * tk.c_smhi +
+                                                (int64_t)(m % tk.c_mlo) * tk.c_smlo);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int n = j * 16 + lr;
+        if (n < nv) {
+          T* q = C + ro + (int64_t)(un.n0 + n) * tk.c_sn;
+          const T v = acc[i][j][r];
+          *q = beta ? *q + v : v;
+        }
+      }
+    }
+}
+
+template <typename T, int RB>
+int launch_skinny(const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
+                  const zmpo_pair* pairs, const void* A, const void* B, void* C, int beta,
+                  hipStream_t s) {
+  hipLaunchKernelGGL((skinny_gemm_kernel<T, RB>), dim3((unsigned)nunits), dim3(256), 0, s, tasks,
+                     units, nunits, pairs, (const T*)A, (const T*)B, (T*)C, beta);
+  return check_hip(hipGetLastError(), "skinny_gemm launch");
+}
+
 // ------------------------------------------------------------- chained GEMMs
 // One workgroup = TM (= 16*RB) rows of one row group.
 //  stage 0: the group's distinct first-stage A blocks (TM rows each) are
@@ -865,7 +979,7 @@ int dispatch_stream(int shape, int kt, const zmpo_sunit* units, int64_t nunits, 
 // M are zeros in atile and are not written to C.
 extern __shared__ __align__(16) unsigned char chain_lds[];
 #ifndef CHAIN_KS
-#define CHAIN_KS 4  // k-steps (of 4) whose operands are loaded before their MFMAs
+#define CHAIN_KS 5  // k-steps (of 4) whose operands are loaded before their MFMAs
 #endif
 
 template <typename T, int RB>
@@ -875,29 +989,63 @@ struct ChainSlice {
 };
 
 // The fields of a pair that the chained stages use (uniform per wave).
+// blk: the first-stage pair's A block (zmpo_cblock index), read when the A
+// operand comes straight from global memory instead of the LDS tile.
 struct ChainPair {
   int64_t b_off;
-  int K, c, ldb;
+  int K, c, ldb, blk;
   __device__ __forceinline__ void get(const zmpo_pair* __restrict__ pairs, int p) {
     const zmpo_pair& q = pairs[p];
     b_off = q.b_off;
     K = q.K;
     c = (int)q.a_off;
     ldb = q.b_sk;
+    blk = q.pad;
   }
+};
+
+// A rows of this lane for a global-memory A block (rows split by the row
+// group's rsplit; rows past M read row 0 and are never stored).
+template <int RB>
+struct ChainRows {
+  int mb[RB], mr[RB];
 };
 
 // Operands of the k-steps [k0, k0 + 4*CHAIN_KS) of one pair, 32-column tile
 // n0.  Only k-steps that start below K are loaded (uniform test); lanes past
 // K inside the last k-step read the zero padding of both operands.  Bases
 // are wave-uniform (scalar registers); each lane adds a fixed 32-bit offset.
-template <typename T, int RB>
+template <typename T, int RB, bool GA>
 __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPair& pr, int n0,
                                            int k0, const T* __restrict__ src,
-                                           const T* __restrict__ Bb, int lane_a, int lr, int lk) {
+                                           const T* __restrict__ Bb, int lane_a, int lr, int lk,
+                                           const T* __restrict__ Ag,
+                                           const zmpo_cblock* __restrict__ blocks,
+                                           const ChainRows<RB>& rows) {
   constexpr int LD = 16 * RB + 1;
   const int lane_b = lk * pr.ldb + lr;
   const T* __restrict__ b0 = Bb + pr.b_off + (int64_t)k0 * pr.ldb + n0;
+  if (GA) {
+    // A block in global memory: lane rows fixed, k clamped into the block
+    // (columns past K meet zero B rows)
+    const zmpo_cblock& bk = blocks[pr.blk];
+    const T* __restrict__ ab = Ag + bk.a_off;
+    int64_t ro[RB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) ro[i] = (int64_t)rows.mb[i] * bk.a_smhi + (int64_t)rows.mr[i] * bk.a_smlo;
+#pragma unroll
+    for (int s = 0; s < CHAIN_KS; ++s) {
+      if (k0 + 4 * s < pr.K) {
+        const T* __restrict__ bs = b0 + 4 * s * pr.ldb;
+        sl.b[s][0] = bs[lane_b];
+        sl.b[s][1] = bs[lane_b + 16];
+        const int64_t ko = (int64_t)min(k0 + 4 * s + lk, pr.K - 1) * bk.a_sk;
+#pragma unroll
+        for (int i = 0; i < RB; ++i) sl.a[s][i] = ab[ro[i] + ko];
+      }
+    }
+    return;
+  }
   const T* __restrict__ a0 = src + (pr.c + k0) * LD;
 #pragma unroll
   for (int s = 0; s < CHAIN_KS; ++s) {
@@ -921,7 +1069,7 @@ __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPai
 // the operands of item i+1 (possibly of the next pair or task) while the
 // MFMAs of item i run; the two operand slices alternate roles, so nothing
 // is copied between them.
-template <typename T, int RB, bool TO_LDS>
+template <typename T, int RB, bool TO_LDS, bool GA>
 struct ChainWalk {
   typedef typename Acc<T>::type accv;
   static constexpr int LD = 16 * RB + 1;
@@ -932,6 +1080,9 @@ struct ChainWalk {
   const T* __restrict__ Bb;
   T* __restrict__ dst;
   T* __restrict__ C;
+  const T* __restrict__ Ag;
+  const zmpo_cblock* __restrict__ blocks;
+  ChainRows<RB> rows;
   int64_t c_base;
   int t1, beta, lane_a, lr, lk;
   // cursor of the item whose operands are being loaded
@@ -942,7 +1093,7 @@ struct ChainWalk {
   // load the item under the cursor into sl; returns its k-step count
   __device__ __forceinline__ int load(ChainSlice<T, RB>& sl) {
     if (p >= tpe) return 0;
-    chain_load<T, RB>(sl, pr, n0, k0, src, Bb, lane_a, lr, lk);
+    chain_load<T, RB, GA>(sl, pr, n0, k0, src, Bb, lane_a, lr, lk, Ag, blocks, rows);
     return min(CHAIN_KS, (pr.K - k0 + 3) / 4);
   }
   // move the cursor to the next item; returns whether the previous item
@@ -1029,14 +1180,18 @@ struct ChainWalk {
   }
 };
 
-template <typename T, int RB, bool TO_LDS>
+template <typename T, int RB, bool TO_LDS, bool GA>
 __device__ __forceinline__ void chain_stage(
     int t0, int t1, const zmpo_task* __restrict__ tasks, const zmpo_pair* __restrict__ pairs,
     const T* __restrict__ src, const T* __restrict__ Bb, T* __restrict__ dst,
     T* __restrict__ C, int64_t c_base, const int (&mb)[RB][4], const int (&mr)[RB][4],
-    const bool (&rok)[RB][4], int beta, int wv, int lr, int lk) {
+    const bool (&rok)[RB][4], int beta, int wv, int lr, int lk, const T* __restrict__ Ag,
+    const zmpo_cblock* __restrict__ blocks, const ChainRows<RB>& rows) {
   typedef typename Acc<T>::type accv;
-  ChainWalk<T, RB, TO_LDS> w;
+  ChainWalk<T, RB, TO_LDS, GA> w;
+  w.Ag = Ag;
+  w.blocks = blocks;
+  w.rows = rows;
   w.t = t0 + wv;
   if (w.t >= t1) return;  // wave-uniform
   w.tasks = tasks;
@@ -1080,7 +1235,7 @@ __device__ __forceinline__ void chain_stage(
   }
 }
 
-template <typename T, int RB>
+template <typename T, int RB, bool ATILE>
 __global__ __launch_bounds__(256) void chain_gemm_kernel(
     const zmpo_cunit* __restrict__ units, int64_t nunits, const zmpo_cgroup* __restrict__ groups,
     const zmpo_cblock* __restrict__ blocks, const zmpo_task* __restrict__ tasks,
@@ -1088,7 +1243,7 @@ __global__ __launch_bounds__(256) void chain_gemm_kernel(
     const T* __restrict__ B2, T* __restrict__ C, int beta, int kcols_max) {
   constexpr int TMv = 16 * RB, LD = TMv + 1;
   T* __restrict__ atile = reinterpret_cast<T*>(chain_lds);
-  T* __restrict__ mid = atile + (int64_t)kcols_max * LD;
+  T* __restrict__ mid = atile + (ATILE ? (int64_t)kcols_max * LD : 0);
   const int lane = threadIdx.x & 63, wv = wave_id();
   const int lr = lane & 15, lk = lane >> 4;
   const int64_t u = xcd_logical_block(blockIdx.x, gridDim.x);
@@ -1098,7 +1253,7 @@ __global__ __launch_bounds__(256) void chain_gemm_kernel(
   const int mv = min(TMv, g.M - un.m0);
 
   // ---- stage 0: A blocks -> atile.  Lane rows: r0 = lane % TM.
-  {
+  if (ATILE) {
     const int r0 = lane & (TMv - 1);
     const bool ok0 = r0 < mv;
     const int m = un.m0 + r0;
@@ -1137,33 +1292,56 @@ __global__ __launch_bounds__(256) void chain_gemm_kernel(
       mb[i][r] = rok[i][r] ? m / g.rsplit : 0;
       mr[i][r] = rok[i][r] ? m % g.rsplit : 0;
     }
-  __syncthreads();
+  // this lane's A rows (MFMA operand layout: row lr of each 16-row block)
+  ChainRows<RB> arows;
+#pragma unroll
+  for (int i = 0; i < RB; ++i) {
+    const int row = i * 16 + lr;
+    const int m = un.m0 + row;
+    arows.mb[i] = row < mv ? m / g.rsplit : 0;
+    arows.mr[i] = row < mv ? m % g.rsplit : 0;
+  }
+  if (ATILE) __syncthreads();
   // ---- stage 1: intermediate columns into LDS
-  chain_stage<T, RB, true>(g.t1_begin, g.t1_end, tasks, pairs, atile, B1 + un.b1_base, mid,
-                           nullptr, 0, mb, mr, rok, 0, wv, lr, lk);
+  chain_stage<T, RB, true, !ATILE>(g.t1_begin, g.t1_end, tasks, pairs, atile, B1 + un.b1_base,
+                                   mid, nullptr, 0, mb, mr, rok, 0, wv, lr, lk, A + un.a_base,
+                                   blocks, arows);
   __syncthreads();
   // ---- stage 2: output blocks
-  chain_stage<T, RB, false>(g.t2_begin, g.t2_end, tasks, pairs, mid, B2 + un.b2_base, nullptr,
-                            C, un.c_base, mb, mr, rok, beta, wv, lr, lk);
+  chain_stage<T, RB, false, false>(g.t2_begin, g.t2_end, tasks, pairs, mid, B2 + un.b2_base,
+                                   nullptr, C, un.c_base, mb, mr, rok, beta, wv, lr, lk,
+                                   nullptr, blocks, arows);
 }
 
-template <typename T, int RB>
-int launch_chain(int kcols_max, int ncols_max, const zmpo_cunit* units, int64_t nunits,
-                 const zmpo_cgroup* groups, const zmpo_cblock* blocks, const zmpo_task* tasks,
-                 const zmpo_pair* pairs, const void* A, const void* B1, const void* B2, void* C,
-                 int beta, hipStream_t s) {
-  const size_t lds = (size_t)(kcols_max + ncols_max) * (16 * RB + 1) * sizeof(T);
+template <typename T, int RB, bool ATILE>
+int launch_chain_t(int kcols_max, int ncols_max, const zmpo_cunit* units, int64_t nunits,
+                   const zmpo_cgroup* groups, const zmpo_cblock* blocks, const zmpo_task* tasks,
+                   const zmpo_pair* pairs, const void* A, const void* B1, const void* B2, void* C,
+                   int beta, hipStream_t s) {
+  const size_t lds = (size_t)((ATILE ? kcols_max : 0) + ncols_max) * (16 * RB + 1) * sizeof(T);
   if (lds > 160 * 1024) return set_error(ZMPO_ERR_ARG, "chaingemm: row tile exceeds 160 KiB of LDS");
   if (lds > 64 * 1024) {
-    int rc = check_hip(hipFuncSetAttribute((const void*)chain_gemm_kernel<T, RB>,
+    int rc = check_hip(hipFuncSetAttribute((const void*)chain_gemm_kernel<T, RB, ATILE>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                        "chaingemm: LDS attribute");
     if (rc) return rc;
   }
-  hipLaunchKernelGGL((chain_gemm_kernel<T, RB>), dim3((unsigned)nunits), dim3(256), lds, s, units,
-                     nunits, groups, blocks, tasks, pairs, (const T*)A, (const T*)B1,
+  hipLaunchKernelGGL((chain_gemm_kernel<T, RB, ATILE>), dim3((unsigned)nunits), dim3(256), lds, s,
+                     units, nunits, groups, blocks, tasks, pairs, (const T*)A, (const T*)B1,
                      (const T*)B2, (T*)C, beta, kcols_max);
   return check_hip(hipGetLastError(), "chain_gemm launch");
+}
+
+template <typename T, int RB>
+int launch_chain(int atile, int kcols_max, int ncols_max, const zmpo_cunit* units,
+                 int64_t nunits, const zmpo_cgroup* groups, const zmpo_cblock* blocks,
+                 const zmpo_task* tasks, const zmpo_pair* pairs, const void* A, const void* B1,
+                 const void* B2, void* C, int beta, hipStream_t s) {
+  if (atile)
+    return launch_chain_t<T, RB, true>(kcols_max, ncols_max, units, nunits, groups, blocks, tasks,
+                                       pairs, A, B1, B2, C, beta, s);
+  return launch_chain_t<T, RB, false>(kcols_max, ncols_max, units, nunits, groups, blocks, tasks,
+                                      pairs, A, B1, B2, C, beta, s);
 }
 
 // ---------------------------------------------------------------- vectors
@@ -1406,10 +1584,29 @@ int zmpo_shapedgemm(int dtype, int shape, const zmpo_task* tasks, const zmpo_uni
   return set_error(ZMPO_ERR_ARG, "shapedgemm: unsupported dtype");
 }
 
-int zmpo_chaingemm(int dtype, int rb, int kcols_max, int ncols_max, const zmpo_cunit* units,
-                   int64_t nunits, const zmpo_cgroup* groups, const zmpo_cblock* blocks,
-                   const zmpo_task* tasks, const zmpo_pair* pairs, const void* A, const void* B1,
-                   const void* B2, void* C, int beta, void* stream) {
+int zmpo_skinnygemm(int dtype, int rb, const zmpo_task* tasks, const zmpo_unit* units,
+                    int64_t nunits, const zmpo_pair* pairs, const void* A, const void* B, void* C,
+                    int beta, void* stream) {
+  if (nunits <= 0) return ZMPO_OK;
+  if (!tasks || !units || !pairs || !A || !B || !C)
+    return set_error(ZMPO_ERR_ARG, "skinnygemm: null pointer");
+  if (nunits > 0x7fffffffLL) return set_error(ZMPO_ERR_ARG, "skinnygemm: too many units");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == ZMPO_F64) {
+    if (rb == 1) return launch_skinny<double, 1>(tasks, units, nunits, pairs, A, B, C, beta, s);
+    if (rb == 2) return launch_skinny<double, 2>(tasks, units, nunits, pairs, A, B, C, beta, s);
+  } else if (dtype == ZMPO_F32) {
+    if (rb == 1) return launch_skinny<float, 1>(tasks, units, nunits, pairs, A, B, C, beta, s);
+    if (rb == 2) return launch_skinny<float, 2>(tasks, units, nunits, pairs, A, B, C, beta, s);
+  }
+  return set_error(ZMPO_ERR_ARG, "skinnygemm: unsupported dtype or row-block count");
+}
+
+int zmpo_chaingemm(int dtype, int rb, int atile, int kcols_max, int ncols_max,
+                   const zmpo_cunit* units, int64_t nunits, const zmpo_cgroup* groups,
+                   const zmpo_cblock* blocks, const zmpo_task* tasks, const zmpo_pair* pairs,
+                   const void* A, const void* B1, const void* B2, void* C, int beta,
+                   void* stream) {
   if (nunits <= 0) return ZMPO_OK;
   if (!units || !groups || !blocks || !tasks || !pairs || !A || !B1 || !B2 || !C)
     return set_error(ZMPO_ERR_ARG, "chaingemm: null pointer");
@@ -1418,11 +1615,11 @@ int zmpo_chaingemm(int dtype, int rb, int kcols_max, int ncols_max, const zmpo_c
     return set_error(ZMPO_ERR_ARG, "chaingemm: kcols_max and ncols_max must be >= 1");
   hipStream_t s = (hipStream_t)stream;
   if (dtype == ZMPO_F64) {
-    if (rb == 1) return launch_chain<double, 1>(kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
-    if (rb == 2) return launch_chain<double, 2>(kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
+    if (rb == 1) return launch_chain<double, 1>(atile, kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
+    if (rb == 2) return launch_chain<double, 2>(atile, kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
   } else if (dtype == ZMPO_F32) {
-    if (rb == 1) return launch_chain<float, 1>(kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
-    if (rb == 2) return launch_chain<float, 2>(kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
+    if (rb == 1) return launch_chain<float, 1>(atile, kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
+    if (rb == 2) return launch_chain<float, 2>(atile, kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
   }
   return set_error(ZMPO_ERR_ARG, "chaingemm: unsupported dtype or row-block count");
 }

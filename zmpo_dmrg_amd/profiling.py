@@ -7,10 +7,11 @@
 
     python -m zmpo_dmrg_amd.profiling launches BENCH_JSON TRACE_CSV
             [--fetch CSV] [--write CSV] [--fetch-scale F] [--write-scale W]
-        per-launch duration and HBM bytes of the sigma launches that bench.py
-        times last (3 repetitions x op chunks x launches, in that order), from
-        rocprofv3 ``-f csv`` kernel-trace / counter-collection files.  Prints a
-        markdown table and a JSON line {launch: bytes_per_launch}.
+        per-launch duration and HBM bytes of the sigma launches (L*psi, the
+        chained W[i]*W[i+1] launch, *R) that bench.py times last (3
+        repetitions x op chunks), from rocprofv3 ``-f csv`` kernel-trace /
+        counter-collection files.  Prints a markdown table and a JSON line
+        {"launches": {name: {"avg_ms", "bytes_per_launch", ...}}}.
 """
 import argparse
 import collections
@@ -18,7 +19,8 @@ import csv
 import json
 import sys
 
-KERNELS = ('shaped_gemm', 'chain_gemm', 'stream_gemm', 'wave_gemm', 'blockgemm')
+KERNELS = ('shaped_gemm', 'shaped_reduce', 'chain_gemm', 'stream_gemm', 'wave_gemm', 'wave_reduce',
+           'skinny_gemm', 'blockgemm')
 
 
 def _copy(gib):
@@ -32,54 +34,77 @@ def _copy(gib):
     print('copied %d bytes x3 (read %d, write %d per pass)' % (n * 8, n * 8, n * 8), flush=True)
 
 
-def _is_sigma_kernel(name):
-    return any(k in name for k in KERNELS)
+def _kind(name):
+    for k in KERNELS:
+        if k in name:
+            return k
+    return None
 
 
-def _trace(path):
-    out = []
+def _dispatches(path, counters=False):
+    """Sigma-kernel dispatches in order: (kind, name, ms or counter dict)."""
+    out = collections.OrderedDict()
     with open(path) as f:
         for r in csv.DictReader(f):
-            if _is_sigma_kernel(r['Kernel_Name']):
-                out.append((r['Kernel_Name'], (int(r['End_Timestamp']) - int(r['Start_Timestamp'])) * 1e-6))
-    return out
-
-
-def _counters(path):
-    agg = collections.OrderedDict()
-    with open(path) as f:
-        for r in csv.DictReader(f):
-            if not _is_sigma_kernel(r['Kernel_Name']):
+            k = _kind(r['Kernel_Name'])
+            if k is None:
                 continue
-            a = agg.setdefault(r['Dispatch_Id'], {})
-            a[r['Counter_Name']] = a.get(r['Counter_Name'], 0.0) + float(r['Counter_Value'])
-    return list(agg.values())
+            if counters:
+                d = out.setdefault(r['Dispatch_Id'], [k, r['Kernel_Name'], {}])
+                d[2][r['Counter_Name']] = d[2].get(r['Counter_Name'], 0.0) + float(r['Counter_Value'])
+            else:
+                out[r['Dispatch_Id']] = [k, r['Kernel_Name'],
+                                         (int(r['End_Timestamp']) - int(r['Start_Timestamp'])) * 1e-6]
+    return list(out.values())
+
+
+def _sigma_blocks(ds):
+    """Group dispatches into sigma applications of the chained path: the
+    L*psi dispatch before each run of chain_gemm dispatches, the run, then
+    the *R dispatch (and its split-K reduction)."""
+    blocks, i = [], 0
+    while i < len(ds):
+        if ds[i][0] != 'chain_gemm':
+            i += 1
+            continue
+        j = i
+        while j < len(ds) and ds[j][0] == 'chain_gemm':
+            j += 1
+        if i == 0 or j >= len(ds):
+            break
+        r_end = j + 1
+        if r_end < len(ds) and ds[r_end][0] == 'shaped_reduce':
+            r_end += 1
+        blocks.append({'L*psi': [ds[i - 1]], 'W[i]*W[i+1] chained': ds[i:j], '*R': ds[j:r_end]})
+        i = r_end
+    return blocks
 
 
 def _launches(args):
     with open(args.bench) as f:
         b = json.load(f)
     names = b['roofline']['stage_names']
-    nch = int(b['config']['chunks'])
-    per = len(names)
-    n = 3 * nch * per
-    tr = _trace(args.trace)[-n:]
-    fe = _counters(args.fetch)[-n:] if args.fetch else None
-    wr = _counters(args.write)[-n:] if args.write else None
-    print('| launch | kernel | avg ms | FETCH GB | WRITE GB | HBM GB/s |')
+    n = 3 * int(b['config']['chunks'])
+    tr = _sigma_blocks(_dispatches(args.trace))[-n:]
+    fe = _sigma_blocks(_dispatches(args.fetch, True))[-n:] if args.fetch else None
+    wr = _sigma_blocks(_dispatches(args.write, True))[-n:] if args.write else None
+    print('| launch | kernels | avg ms | FETCH GB | WRITE GB | HBM GB/s |')
     print('|---|---|---|---|---|---|')
     res = {}
-    for s, nm in enumerate(names):
-        idx = list(range(s, n, per))
-        ms = sum(tr[i][1] for i in idx) / len(idx)
-        fgb = sum(fe[i]['FETCH_SIZE'] for i in idx) * 1024 * args.fetch_scale / len(idx) / 1e9 if fe else None
-        wgb = sum(wr[i]['WRITE_SIZE'] for i in idx) * 1024 * args.write_scale / len(idx) / 1e9 if wr else None
-        tot = (fgb or 0) + (wgb or 0)
-        res[nm] = tot * 1e9 if fe and wr else None
-        kn = tr[idx[0]][0].split('(')[0].replace('void ', '').replace('(anonymous namespace)::', '')
-        print('| %s | `%s` | %.3f | %s | %s | %s |' % (
-            nm, kn, ms, '%.2f' % fgb if fe else '-', '%.2f' % wgb if wr else '-',
-            '%.0f' % (tot / ms * 1e3) if fe and wr else '-'))
+    for nm in names:
+        ms = sum(sum(d[2] for d in blk[nm]) for blk in tr) / len(tr)
+        kn = sorted({d[1].replace('void ', '').replace('(anonymous namespace)::', '').split('(')[0]
+                     for blk in tr for d in blk[nm]})
+        fgb = (sum(sum(d[2].get('FETCH_SIZE', 0.0) for d in blk[nm]) for blk in fe) * 1024
+               * args.fetch_scale / len(fe) / 1e9) if fe else None
+        wgb = (sum(sum(d[2].get('WRITE_SIZE', 0.0) for d in blk[nm]) for blk in wr) * 1024
+               * args.write_scale / len(wr) / 1e9) if wr else None
+        tot = (fgb or 0.0) + (wgb or 0.0)
+        res[nm] = {'avg_ms': ms, 'bytes_per_launch': tot * 1e9 if (fe and wr) else None,
+                   'fetch_gb': fgb, 'write_gb': wgb}
+        print('| %s | %s | %.3f | %s | %s | %s |' % (
+            nm, ', '.join('`%s`' % k for k in kn), ms, '%.2f' % fgb if fe else '-',
+            '%.2f' % wgb if wr else '-', '%.0f' % (tot / ms * 1e3) if (fe and wr) else '-'))
     print(json.dumps({'launches': res}))
 
 

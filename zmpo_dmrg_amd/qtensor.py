@@ -562,6 +562,35 @@ STREAM_KT = int(os.environ.get('ZMPO_KT', '32'))
 STREAM_UPW = int(os.environ.get('ZMPO_UPW', '0'))
 
 
+SINGLE = 1 << 30   # "mlo" of a single-level index map (no index reaches it)
+
+
+def pair_tasks(tasks, npairs):
+    """Task index of every pair from the tasks' [p_begin, p_end) ranges (any task order)."""
+    cnt = (tasks['p_end'] - tasks['p_begin']).astype(np.int64)
+    out = np.full(npairs, -1, np.int64)
+    if cnt.sum():
+        pos = np.repeat(tasks['p_begin'].astype(np.int64), cnt) + (
+            np.arange(int(cnt.sum())) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+        out[pos] = np.repeat(np.arange(len(tasks)), cnt)
+    return out
+
+
+def collapse_maps(tasks, pairs):
+    """Copies of the arrays with every 2-level index map that is really one
+    stride (hi stride == lo extent * lo stride) turned into a single-level
+    map, which the kernels apply without a division."""
+    t, p = tasks.copy(), pairs.copy()
+    for arr, lo, hi, sl in ((p, 'a_mlo', 'a_smhi', 'a_smlo'), (p, 'b_nlo', 'b_snhi', 'b_snlo'),
+                            (t, 'c_mlo', 'c_smhi', 'c_smlo')):
+        if len(arr) == 0:
+            continue
+        one = arr[hi].astype(np.int64) == arr[lo].astype(np.int64) * arr[sl]
+        arr[lo][one] = SINGLE
+        arr[hi][one] = 0
+    return t, p
+
+
 def _max_row_offset(lo, shi, slo, extent):
     """Largest in-block row offset (r // lo) * shi + (r % lo) * slo, r < extent."""
     lo = lo.astype(np.int64)
@@ -591,8 +620,10 @@ def stream_upw(nunits):
 class Program:
     """Task/pair arrays of one grouped-GEMM launch (uploaded on first use)."""
 
-    def __init__(self, tasks, pairs, reuse_key=None):
+    def __init__(self, tasks, pairs, reuse_key=None, prefer=None):
         self.reuse_key = reuse_key
+        self.prefer = prefer   # kernel to use when eligible (None: the library default)
+        tasks, pairs = collapse_maps(tasks, pairs)
         self.ntasks = len(tasks)
         self.npairs = len(pairs)
         self.ntiles = finish_tasks(tasks) if self.ntasks else 0
@@ -611,6 +642,33 @@ class Program:
             _max_row_offset(pairs['a_mlo'], pairs['a_smhi'], pairs['a_smlo'], tasks['M'][rows]),
             _max_row_offset(pairs['b_nlo'], pairs['b_snhi'], pairs['b_snlo'], tasks['N'][rows]))
             < 2 ** 31 - 1)
+
+    def skinny_ok(self):
+        """Whether zmpo_skinnygemm applies: K-contiguous single-level A rows."""
+        if self.npairs == 0:
+            return False
+        t, p = self.tasks_host, self.pairs_host
+        rows = np.repeat(np.arange(self.ntasks), (t['p_end'] - t['p_begin']).astype(np.int64))
+        return bool(np.all(p['a_sk'] == 1) and np.all(p['a_mlo'] >= t['M'][rows]))
+
+    def skinny_units(self, rb):
+        """zmpo_unit records of the skinny kernel: (64*rb)-row strips x 64-column
+        chunks of every task, each over all of the task's pairs."""
+        t = self.tasks_host
+        tm = 64 * rb
+        ns = np.maximum((t['M'].astype(np.int64) + tm - 1) // tm, 1)
+        nc = np.maximum((t['N'].astype(np.int64) + 63) // 64, 1)
+        per = ns * nc
+        ti = np.repeat(np.arange(self.ntasks), per)
+        w = np.arange(int(per.sum())) - np.repeat(np.cumsum(per) - per, per)
+        u = np.zeros(len(ti), _lib.UNIT_DTYPE)
+        u['task'] = ti
+        u['m0'] = (w // nc[ti]) * tm
+        u['n0'] = (w % nc[ti]) * 64
+        u['p_begin'] = t['p_begin'][ti]
+        u['p_end'] = t['p_end'][ti]
+        u['slot'] = -1
+        return u
 
     def alg_bytes(self, elsize=8, beta=0):
         """Algorithmic HBM bytes of one launch: every distinct A and B block
@@ -635,6 +693,9 @@ class Program:
         """Device units / split-K groups for the wave or shaped kernel (built once each)."""
         if self._sched is None:
             self._sched = {}
+        if kernel == 'skinny' and kernel not in self._sched:
+            u = self.skinny_units(SKINNY_RB)
+            self._sched[kernel] = {'units': dv.upload_bytes(u), 'nunits': len(u), 'rb': SKINNY_RB}
         if kernel not in self._sched:
             tile = None if kernel == 'wave' else SHAPES[self.shape]
             units, groups, nslots = make_schedule(self.tasks_host, self.pairs_host, self.rb,
@@ -666,6 +727,7 @@ def _workspace(n, dtype):
 
 
 KERNEL = os.environ.get('ZMPO_KERNEL', 'shaped')
+SKINNY_RB = int(os.environ.get('ZMPO_SKINNY_RB', '1'))
 
 
 def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None):
@@ -675,10 +737,13 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None)
     software pipeline, 'shaped' one LDS-staged tile per workgroup, 'wave' one
     wave per small tile, all with deterministic split-K;
     'tile' runs the 64x64 LDS-tiled workgroup kernel."""
-    kernel = KERNEL if kernel is None else kernel
     if isinstance(prog, dict):
         prog = Program(prog['tasks'], prog['pairs'])
+    if kernel is None:
+        kernel = prog.prefer if prog.prefer else KERNEL
     if kernel == 'stream' and not prog.small_blocks:
+        kernel = 'shaped'
+    if kernel == 'skinny' and not prog.skinny_ok():
         kernel = 'shaped'
     if prog.ntasks == 0:
         return
@@ -690,7 +755,12 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None)
                                     prog.ntiles, dv.ptr(A), dv.ptr(B), dv.ptr(C), beta, st))
         return
     sc = prog.schedule(kernel)
-    work = _workspace(sc['nslots'] * sc['slot_size'], dtype) if sc['nslots'] else None
+    work = _workspace(sc['nslots'] * sc['slot_size'], dtype) if sc.get('nslots') else None
+    if kernel == 'skinny':
+        _lib.check(L.zmpo_skinnygemm(dtype, sc['rb'], dv.ptr(tasks), dv.ptr(sc['units']),
+                                     sc['nunits'], dv.ptr(pairs), dv.ptr(A), dv.ptr(B), dv.ptr(C),
+                                     beta, st))
+        return
     if kernel == 'stream':
         _lib.check(L.zmpo_streamgemm(dtype, prog.shape, sc['kt'], dv.ptr(sc['units']),
                                      sc['nunits'], sc['upw'], dv.ptr(tasks), dv.ptr(sc['groups']),
@@ -819,12 +889,13 @@ def chain_structure(first, second, mid_key, out_key, mid_off, b1_map, b2_map):
         for q in range(lo, hi):
             ao = int(pairs['a_off'][q])
             if ao not in seen:
-                seen[ao] = int(kcols[gi])
+                seen[ao] = (int(kcols[gi]), len(blocks))
                 blocks.append((ao, int(pairs['a_smhi'][q]), int(pairs['a_smlo'][q]),
                                int(pairs['a_sk'][q]), int(pairs['K'][q]), int(kcols[gi]),
                                int(pairs['a_mlo'][q]), 0))
                 kcols[gi] += (int(pairs['K'][q]) + 3) // 4 * 4
-            pairs['a_off'][q] = seen[ao]
+            pairs['a_off'][q] = seen[ao][0]
+            pairs['pad'][q] = seen[ao][1]    # the A block, for reads from global memory
     ab[len(keys)] = len(blocks)
     blocks = np.array(blocks, dtype=_lib.CBLOCK_DTYPE) if blocks else np.zeros(0, _lib.CBLOCK_DTYPE)
     groups = np.zeros(len(keys), _lib.CGROUP_DTYPE)
@@ -856,10 +927,21 @@ def chain_structure(first, second, mid_key, out_key, mid_off, b1_map, b2_map):
             blocks['a_smlo'] + (blocks['K'].astype(np.int64) - 1) * blocks['a_sk']
         if big.max() >= 2 ** 31:
             raise NotImplementedError('chained plans: an A block exceeds 2^31 elements')
+    # inside each stage of each group, costly tasks first: the waves take
+    # tasks round-robin, so this balances them (pair ranges are unchanged)
+    kp = (pairs['K'].astype(np.int64) + 3) // 4
+    prow = np.repeat(np.arange(len(tasks)), (tasks['p_end'] - tasks['p_begin']).astype(np.int64))
+    cost = np.bincount(prow, weights=kp, minlength=len(tasks)) * ((tasks['N'] + 31) // 32)
+    for lo, hi in zip(np.concatenate([groups['t1_begin'], groups['t2_begin']]),
+                      np.concatenate([groups['t1_end'], groups['t2_end']])):
+        if hi - lo > 1:
+            o = lo + np.argsort(-cost[lo:hi], kind='stable')
+            tasks[lo:hi] = tasks[o]
     return groups, tasks, pairs, blocks
 
 
 CHAIN_CLASSES = (96, 160, 256)   # LDS size classes (A-tile + intermediate columns)
+CHAIN_ATILE = os.environ.get('ZMPO_CHAIN_ATILE', '0')   # '1', '0' or 'auto' (by class)
 
 
 class ChainProgram:
@@ -889,6 +971,8 @@ class ChainProgram:
                 g[f] += bbase
             t['p_begin'] += pbase
             t['p_end'] += pbase
+            p = p.copy()
+            p['pad'] += bbase
             gbase[c] = ng
             self._gspan[c] = (ng, len(g))
             ng += len(g)
@@ -906,7 +990,7 @@ class ChainProgram:
         uparts = []
         self.flops = 0
         t, p = self.tasks_host, self.pairs_host
-        prow = np.repeat(np.arange(len(t)), (t['p_end'] - t['p_begin']).astype(np.int64))
+        prow = pair_tasks(t, len(p))
         pfl = 2 * t['M'][prow].astype(np.int64) * t['N'][prow] * p['K'].astype(np.int64)
         tfl = np.bincount(prow, weights=pfl, minlength=len(t)) if len(p) else np.zeros(len(t))
         self.instance_cases = [c for c, *_ in instances]
@@ -939,8 +1023,11 @@ class ChainProgram:
                 continue
             us = self.units_host[sel]
             gsel = us['group']
+            # the widest class reads A from global memory so that its LDS
+            # (the intermediate only) still allows several workgroups per CU
+            atile = {'1': 1, '0': 0}.get(CHAIN_ATILE, int(k < len(CHAIN_CLASSES)))
             self.launches.append({'units': us, 'kmax': int(max(1, g['kcols'][gsel].max())),
-                                  'nmax': int(max(1, g['ncols'][gsel].max()))})
+                                  'nmax': int(max(1, g['ncols'][gsel].max())), 'atile': atile})
         self._dev = None
 
     def with_rb(self, rb):
@@ -960,18 +1047,19 @@ class ChainProgram:
         u = self.units_host
         rows = np.minimum(16 * self.rb, g['M'][u['group']].astype(np.int64) - u['m0'])
         tot = int(np.sum(rows * cols[u['group']]))
-        prow = np.repeat(np.arange(len(t)), (t['p_end'] - t['p_begin']).astype(np.int64))
+        prow = pair_tasks(t, len(p))
         bsz = t['N'][prow].astype(np.int64) * p['K']
         stage2 = np.zeros(len(p), np.int64)
         for gg in g:
-            if gg['t2_end'] > gg['t2_begin']:
-                stage2[t['p_begin'][gg['t2_begin']]:t['p_end'][gg['t2_end'] - 1]] = 1
+            for ti in range(gg['t2_begin'], gg['t2_end']):
+                stage2[t['p_begin'][ti]:t['p_end'][ti]] = 1
         for c in self.instance_cases:
             gl, gn = self._gspan[c]
             if gn == 0:
                 continue
-            lo = t['p_begin'][g['t1_begin'][gl]]
-            hi = t['p_end'][g['t2_end'][gl + gn - 1] - 1]
+            tr = slice(g['t1_begin'][gl], g['t2_end'][gl + gn - 1])
+            lo = int(t['p_begin'][tr].min())
+            hi = int(t['p_end'][tr].max())
             key = p['b_off'][lo:hi] * 2 + stage2[lo:hi]
             _, ib = np.unique(key, return_index=True)
             tot += int(bsz[lo:hi][ib].sum())
@@ -993,7 +1081,7 @@ def run_chain(prog, A, B1, B2, C, dtype=_lib.F64, beta=0, stream=None):
     st = dv.stream() if stream is None else stream
     L = _lib.lib()
     for l_, u in zip(prog.launches, units):
-        _lib.check(L.zmpo_chaingemm(dtype, prog.rb, l_['kmax'], l_['nmax'], dv.ptr(u),
+        _lib.check(L.zmpo_chaingemm(dtype, prog.rb, l_['atile'], l_['kmax'], l_['nmax'], dv.ptr(u),
                                     len(l_['units']), dv.ptr(g), dv.ptr(b), dv.ptr(t), dv.ptr(p),
                                     dv.ptr(A), dv.ptr(B1), dv.ptr(B2), dv.ptr(C), beta, st))
 
