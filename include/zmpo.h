@@ -166,6 +166,16 @@ int zmpo_streamgemm(int dtype, int shape, int kt, const zmpo_sunit* units, int64
                     int64_t ngroups, const zmpo_pair* pairs, const void* A, const void* B,
                     void* C, void* work, int beta, void* stream);
 
+/* Grouped GEMM for small output blocks with a very long summed K and both
+ * operands K-contiguous (sigma stage top3 x R of HVecQt, mpo_dmrg_kernelQt.
+ * py:297-306): the zmpo_unit / split-K schedule of zmpo_shapedgemm with the
+ * 32x32 tile (ZMPO_TILE_32x32), work holding 32*32 elements per slot.  The
+ * four waves of a workgroup split each pair's k range and their partial
+ * tiles are summed in a fixed order. */
+int zmpo_dotgemm(int dtype, const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
+                 const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
+                 const void* A, const void* B, void* C, void* work, int beta, void* stream);
+
 /* Grouped GEMM for tall, narrow blocks (sigma stage L x psi of HVecQt,
  * mpo_dmrg_kernelQt.py:270-276, with the same task/pair semantics as
  * zmpo_blockgemm): one workgroup per zmpo_unit = rows [m0, m0 + 64*rb) x

@@ -49,7 +49,7 @@ def emulate_wave(prog, A, B, C, dtype=0, beta=0, stream=None, kernel=None):
         return emulate(prog, A, B, C, dtype, beta)
     A, B, C = _np(A), _np(B), _np(C)
     rb = prog.rb
-    tile = None if kernel == 'wave' else SHAPES[prog.shape]
+    tile = None if kernel == 'wave' else (32, 32) if kernel == 'dot' else SHAPES[prog.shape]
     TMs, TNs = (16 * rb, 16) if tile is None else tile
     units, groups, nslots = make_schedule(prog.tasks_host, prog.pairs_host, rb,
                                           reuse_key=getattr(prog, 'reuse_key', None), tile=tile)

@@ -24,7 +24,7 @@ def _dense_program(M, N, K, trans_a=False, trans_b=False):
 @pytest.mark.parametrize('shape', [(64, 64, 16), (37, 90, 33), (130, 7, 200), (1, 1, 1),
                                    (5, 9, 5000)])
 @pytest.mark.parametrize('ta,tb', [(False, False), (True, False), (False, True)])
-@pytest.mark.parametrize('kernel', ['wave', 'tile', 'shaped', 'stream', 'skinny'])
+@pytest.mark.parametrize('kernel', ['wave', 'tile', 'shaped', 'stream', 'skinny', 'dot'])
 def test_blockgemm_dense_f64(gpu, shape, ta, tb, kernel):
     M, N, K = shape
     rng = np.random.default_rng(M * N + K)
@@ -78,6 +78,7 @@ def test_wavegemm_split_k_random_programs(gpu):
     emulate(prog, A, B, want)
     configs = [('wave', r, 0, 0, 0) for r in (1, 2, 4)] + [('shaped', 1, s, 0, 0) for s in pq.SHAPES]
     configs += [('stream', 1, s, kt, upw) for s in pq.SHAPES for kt in (16, 32) for upw in (1, 3, 64)]
+    configs += [('dot', 1, 2, 0, 0)]
     for kernel, rb, shape, kt, upw in configs:
         prog = pq.Program(tasks, pairs)
         prog.rb, prog.shape, prog.kt, prog.upw = rb, shape, kt or None, upw or None
@@ -86,7 +87,7 @@ def test_wavegemm_split_k_random_programs(gpu):
         assert np.abs(dv.download(C) - want).max() < 1e-10, (kernel, rb, shape, kt, upw)
 
 
-@pytest.mark.parametrize('kernel', ['wave', 'tile', 'shaped', 'stream', 'skinny'])
+@pytest.mark.parametrize('kernel', ['wave', 'tile', 'shaped', 'stream', 'skinny', 'dot'])
 def test_blockgemm_beta_accumulates(gpu, kernel):
     M = N = K = 20
     rng = np.random.default_rng(2)
@@ -124,7 +125,8 @@ def test_blockgemm_real_plans_match_model(gpu, case4):
             A, B = rng.standard_normal(na), rng.standard_normal(nb)
             want = np.zeros(nc)
             emulate(pg, A, B, want)
-            for kernel, upw in (('shaped', None), ('stream', None), ('stream', 64), ('skinny', None)):
+            for kernel, upw in (('shaped', None), ('stream', None), ('stream', 64), ('skinny', None),
+                                ('dot', None)):
                 pg.upw = upw
                 pg._sched = None
                 C = dv.zeros(nc, 0)

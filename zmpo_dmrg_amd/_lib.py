@@ -40,7 +40,7 @@ SUNIT_DTYPE = np.dtype([('c_off', '<i8'), ('c_smhi', '<i8'), ('c_mlo', '<i4'), (
 TILE = 64
 
 EXPORTS = ['zmpo_last_error', 'zmpo_abi_version', 'zmpo_struct_sizes', 'zmpo_symAllowed',
-           'zmpo_symAllowedInt', 'zmpo_blockgemm', 'zmpo_wavegemm', 'zmpo_shapedgemm', 'zmpo_streamgemm', 'zmpo_chaingemm', 'zmpo_skinnygemm', 'zmpo_dots', 'zmpo_dots_workspace',
+           'zmpo_symAllowedInt', 'zmpo_blockgemm', 'zmpo_wavegemm', 'zmpo_shapedgemm', 'zmpo_streamgemm', 'zmpo_chaingemm', 'zmpo_skinnygemm', 'zmpo_dotgemm', 'zmpo_dots', 'zmpo_dots_workspace',
            'zmpo_lincomb', 'zmpo_precond', 'zmpo_gather', 'zmpo_scatter', 'zmpo_convert']
 
 
@@ -68,6 +68,7 @@ def lib():
     L.zmpo_blockgemm.argtypes = [i32, vp, i64, vp, i64, vp, vp, vp, i32, vp]
     L.zmpo_wavegemm.argtypes = [i32, i32, vp, vp, i64, vp, i64, vp, vp, vp, vp, vp, i32, vp]
     L.zmpo_shapedgemm.argtypes = [i32, i32, vp, vp, i64, vp, i64, vp, vp, vp, vp, vp, i32, vp]
+    L.zmpo_dotgemm.argtypes = [i32, vp, vp, i64, vp, i64, vp, vp, vp, vp, vp, i32, vp]
     L.zmpo_skinnygemm.argtypes = [i32, i32, vp, vp, i64, vp, vp, vp, vp, i32, vp]
     L.zmpo_chaingemm.argtypes = [i32, i32, i32, i32, i32, vp, i64, vp, vp, vp, vp, vp, vp, vp,
                                  vp, i32, vp]

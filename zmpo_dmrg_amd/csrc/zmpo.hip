@@ -852,12 +852,12 @@ int dispatch_stream(int shape, int kt, const zmpo_sunit* units, int64_t nunits, 
 // Tall blocks with few columns (sigma stage 1: M = bra x MPO-bond rows in
 // the hundreds, N and K = sector dims of 5-60).  One workgroup = a strip of
 // 64*RB rows x up to 64 columns of one output block; each wave owns 16*RB
-// rows.  B (K x N, small) is staged once per 64-deep K chunk into LDS with
-// zero padding; A rows (single-level, K-contiguous) go straight from
-// global memory into the MFMA operand registers, all k-steps of a chunk
-// issued before their MFMAs, with the row clamped into the block and k
-// clamped to K-1 (those products meet zero B rows or are not stored), so
-// the inner loop has no bounds tests.  Tasks may have several pairs; each
+// rows and works alone (no LDS, no barriers).  Operands go straight from
+// global memory into the MFMA registers -- A rows K-contiguous, B rows
+// N-contiguous (16 lanes = one 128-byte segment) -- all k steps of a 4*SK
+// slice issued before their MFMAs; rows and columns are clamped into the
+// block (those results are not stored) and A is zeroed past K, so the
+// products need no other bounds tests.  Tasks may have several pairs; each
 // output element is written once.
 template <typename T, int RB>
 __global__ __launch_bounds__(256) void skinny_gemm_kernel(
@@ -865,19 +865,18 @@ __global__ __launch_bounds__(256) void skinny_gemm_kernel(
     const zmpo_pair* __restrict__ pairs, const T* __restrict__ A, const T* __restrict__ B,
     T* __restrict__ C, int beta) {
   typedef typename Acc<T>::type accv;
-  constexpr int KC = 64, NC = 64;
-  __shared__ T Bs[KC][NC + 1];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wv = wave_id();
+  constexpr int SK = 4;  // k steps per slice
+  const int lane = threadIdx.x & 63, wv = wave_id();
   const int lr = lane & 15, lk = lane >> 4;
   const int64_t u = xcd_logical_block(blockIdx.x, gridDim.x);
   if (u >= nunits) return;  // workgroup-uniform
   const zmpo_unit un = units[u];
   const zmpo_task& tk = tasks[un.task];
   const int M = tk.M, N = tk.N;
-  const int nv = min(NC, N - un.n0);
-  const int nt = (nv + 15) >> 4;  // 16-column MFMA tiles in use (uniform)
   const int rbase = un.m0 + wv * 16 * RB;
+  if (rbase >= M) return;  // wave-uniform: this wave's rows are all past the block
+  const int nv = min(64, N - un.n0);
+  const int nt = (nv + 15) >> 4;  // 16-column MFMA tiles in use (uniform)
   accv acc[RB][4];
 #pragma unroll
   for (int i = 0; i < RB; ++i)
@@ -886,46 +885,46 @@ __global__ __launch_bounds__(256) void skinny_gemm_kernel(
   for (int p = un.p_begin; p < un.p_end; ++p) {
     const zmpo_pair& pr = pairs[p];
     const int K = pr.K;
-    // this lane's A rows (clamped into the block)
-    int64_t ro[RB];
-#pragma unroll
-    for (int i = 0; i < RB; ++i) ro[i] = (int64_t)min(rbase + i * 16 + lr, M - 1) * pr.a_smlo;
-    const T* __restrict__ a = A + pr.a_off;
     const T sc = (T)pr.scale;
-    const bool bone = pr.b_nlo >= N;
-    for (int kc = 0; kc < K; kc += KC) {
-      const int kv = min(KC, K - kc);
-      __syncthreads();  // the previous chunk's readers are done with Bs
-      for (int e = tid; e < KC * NC; e += 256) {
-        const int k = e / NC, n = e % NC;
-        T v = (T)0;
-        if (k < kv && n < nv) {
-          const int nn = un.n0 + n;
-          const int64_t co = bone ? (int64_t)nn * pr.b_snlo
-                                  : (int64_t)(nn / pr.b_nlo) * pr.b_snhi + (int64_t)(nn % pr.b_nlo) * pr.b_snlo;
-          v = B[pr.b_off + (int64_t)(kc + k) * pr.b_sk + co] * sc;
+    const T* __restrict__ a = A + pr.a_off;
+    const T* __restrict__ b = B + pr.b_off;
+    int64_t ro[RB], co[4];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int m = min(rbase + i * 16 + lr, M - 1);
+      ro[i] = pr.a_mlo >= M ? (int64_t)m * pr.a_smlo
+                            : (int64_t)(m / pr.a_mlo) * pr.a_smhi + (int64_t)(m % pr.a_mlo) * pr.a_smlo;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = min(un.n0 + j * 16 + lr, N - 1);
+      co[j] = pr.b_nlo >= N ? (int64_t)n * pr.b_snlo
+                            : (int64_t)(n / pr.b_nlo) * pr.b_snhi + (int64_t)(n % pr.b_nlo) * pr.b_snlo;
+    }
+    for (int k0 = 0; k0 < K; k0 += 4 * SK) {
+      T av[SK][RB], bv[SK][4];
+#pragma unroll
+      for (int s = 0; s < SK; ++s) {
+        const int k = k0 + 4 * s + lk;
+        const bool kok = k < K;
+        const int64_t kc = min(k, K - 1);
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+          const T x = a[ro[i] + kc * pr.a_sk];
+          av[s][i] = kok ? x * sc : (T)0;
         }
-        Bs[k][n] = v;
-      }
-      __syncthreads();
-      T av[16][RB];
 #pragma unroll
-      for (int s = 0; s < 16; ++s) {
-        if (4 * s < kv) {
-          const int k = min(kc + 4 * s + lk, K - 1);
-#pragma unroll
-          for (int i = 0; i < RB; ++i) av[s][i] = a[ro[i] + k];
-        }
+        for (int j = 0; j < 4; ++j)
+          if (j < nt) bv[s][j] = b[co[j] + kc * pr.b_sk];
       }
 #pragma unroll
-      for (int s = 0; s < 16; ++s) {
-        if (4 * s >= kv) break;  // uniform
+      for (int s = 0; s < SK; ++s) {
+        if (k0 + 4 * s >= K) break;  // uniform
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           if (j >= nt) break;  // uniform
-          const T bv = Bs[4 * s + lk][j * 16 + lr];
 #pragma unroll
-          for (int i = 0; i < RB; ++i) acc[i][j] = Acc<T>::mfma(av[s][i], bv, acc[i][j]);
+          for (int i = 0; i < RB; ++i) acc[i][j] = Acc<T>::mfma(av[s][i], bv[s][j], acc[i][j]);
         }
       }
     }
@@ -960,6 +959,134 @@ int launch_skinny(const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits
   hipLaunchKernelGGL((skinny_gemm_kernel<T, RB>), dim3((unsigned)nunits), dim3(256), 0, s, tasks,
                      units, nunits, pairs, (const T*)A, (const T*)B, (T*)C, beta);
   return check_hip(hipGetLastError(), "skinny_gemm launch");
+}
+
+// ---------------------------------------------------------------- dot GEMMs
+// Both operands K-contiguous (A rows and B columns are runs of K elements),
+// small M and N, very long summed K (sigma stage top3 x R: sigma block rows
+// = bra states, columns = right states, K = all ops x (MPO bond x right
+// bond)).  One workgroup = one 32x32 output tile and one chunk of its pairs
+// (the zmpo_unit schedule of zmpo_shapedgemm with tile 32x32 and split-K
+// slots).  The four waves split every pair's k range (wave w takes the
+// 4-deep k steps w, w+4, ...), each holding a 2x2 grid of 16x16 MFMA tiles;
+// operands go straight from global memory into MFMA registers (rows and
+// columns clamped into the block, A zeroed past K), and the four partial
+// tiles are summed through LDS in wave order at the end.
+template <typename T>
+__global__ __launch_bounds__(256) void dot_gemm_kernel(
+    const zmpo_task* __restrict__ tasks, const zmpo_unit* __restrict__ units, int64_t nunits,
+    const zmpo_pair* __restrict__ pairs, const T* __restrict__ A, const T* __restrict__ B,
+    T* __restrict__ C, T* __restrict__ work, int beta) {
+  typedef typename Acc<T>::type accv;
+  constexpr int TMv = 32, TNv = 32, DS = 4;  // DS: k steps loaded before their MFMAs
+  __shared__ T red[3][TMv][TNv + 1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = wave_id();
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t u = xcd_logical_block(blockIdx.x, gridDim.x);
+  if (u >= nunits) return;  // workgroup-uniform
+  const zmpo_unit un = units[u];
+  const zmpo_task& tk = tasks[un.task];
+  const int M = tk.M, N = tk.N;
+  accv acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) acc[i][0] = acc[i][1] = accv{0, 0, 0, 0};
+  for (int p = un.p_begin; p < un.p_end; ++p) {
+    const zmpo_pair& pr = pairs[p];
+    const int K = pr.K;
+    const T sc = (T)pr.scale;
+    // lane rows / columns (clamped into the block; results past M, N unused)
+    int64_t ar[2], bc[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int m = min(un.m0 + i * 16 + lr, M - 1);
+      ar[i] = pr.a_mlo >= M ? (int64_t)m * pr.a_smlo
+                            : (int64_t)(m / pr.a_mlo) * pr.a_smhi + (int64_t)(m % pr.a_mlo) * pr.a_smlo;
+      const int n = min(un.n0 + i * 16 + lr, N - 1);
+      bc[i] = pr.b_nlo >= N ? (int64_t)n * pr.b_snlo
+                            : (int64_t)(n / pr.b_nlo) * pr.b_snhi + (int64_t)(n % pr.b_nlo) * pr.b_snlo;
+    }
+    const T* __restrict__ a = A + pr.a_off;
+    const T* __restrict__ b = B + pr.b_off;
+    // this wave's k steps: 4*(wv + 4*j) .. + 3
+    for (int k0 = 4 * wv; k0 < K; k0 += 16 * DS) {
+      T av[DS][2], bv[DS][2];
+#pragma unroll
+      for (int s = 0; s < DS; ++s) {
+        const int k = k0 + 16 * s + lk;
+        const bool kok = k < K;
+        const int kc = min(k, K - 1);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const T x = a[ar[i] + (int64_t)kc * pr.a_sk];
+          av[s][i] = kok ? x * sc : (T)0;
+          bv[s][i] = b[bc[i] + (int64_t)kc * pr.b_sk];
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < DS; ++s) {
+        if (k0 + 16 * s >= K) break;  // uniform
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[i][j] = Acc<T>::mfma(av[s][i], bv[s][j], acc[i][j]);
+      }
+    }
+  }
+  // sum the four waves' partial tiles in wave order: waves 1-3 park theirs
+  // in LDS, wave 0 adds them to its own and writes the tile
+  if (wv > 0) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          red[wv - 1][i * 16 + Acc<T>::row(lk, r)][j * 16 + lr] = acc[i][j][r];
+  }
+  __syncthreads();
+  if (wv > 0) return;
+  const int mv = min(TMv, M - un.m0), nv = min(TNv, N - un.n0);
+  const bool cone = tk.c_mlo >= M;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = i * 16 + Acc<T>::row(lk, r);
+      const int m = un.m0 + row;
+      const int64_t ro = tk.c_off + (cone ? (int64_t)m * tk.c_smlo
+                                          : (int64_t)(m / tk.c_mlo) * tk.c_smhi +
+                                                (int64_t)(m % tk.c_mlo) * tk.c_smlo);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = j * 16 + lr;
+        const T v = ((acc[i][j][r] + red[0][row][col]) + red[1][row][col]) + red[2][row][col];
+        if (un.slot >= 0) {
+          work[(int64_t)un.slot * (TMv * TNv) + row * TNv + col] = v;
+        } else if (row < mv && col < nv) {
+          T* q = C + ro + (int64_t)(un.n0 + col) * tk.c_sn;
+          *q = beta ? *q + v : v;
+        }
+      }
+    }
+}
+
+template <typename T>
+int launch_dot(const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
+               const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs, const void* A,
+               const void* B, void* C, void* work, int beta, hipStream_t s) {
+  if (nunits > 0) {
+    hipLaunchKernelGGL((dot_gemm_kernel<T>), dim3((unsigned)nunits), dim3(256), 0, s, tasks, units,
+                       nunits, pairs, (const T*)A, (const T*)B, (T*)C, (T*)work, beta);
+    int rc = check_hip(hipGetLastError(), "dot_gemm launch");
+    if (rc) return rc;
+  }
+  if (ngroups > 0) {
+    hipLaunchKernelGGL((shaped_reduce_kernel<T, 32, 32>), dim3((unsigned)ngroups), dim3(256), 0, s,
+                       tasks, groups, ngroups, (const T*)work, (T*)C, beta);
+    return check_hip(hipGetLastError(), "dot_reduce launch");
+  }
+  return ZMPO_OK;
 }
 
 // ------------------------------------------------------------- chained GEMMs
@@ -1582,6 +1709,22 @@ int zmpo_shapedgemm(int dtype, int shape, const zmpo_task* tasks, const zmpo_uni
   if (dtype == ZMPO_F32)
     return dispatch_shaped<float>(shape, tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
   return set_error(ZMPO_ERR_ARG, "shapedgemm: unsupported dtype");
+}
+
+int zmpo_dotgemm(int dtype, const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
+                 const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
+                 const void* A, const void* B, void* C, void* work, int beta, void* stream) {
+  if (nunits > 0 && (!tasks || !units || !pairs || !A || !B || !C))
+    return set_error(ZMPO_ERR_ARG, "dotgemm: null pointer");
+  if (ngroups > 0 && (!groups || !work)) return set_error(ZMPO_ERR_ARG, "dotgemm: null workspace");
+  if (nunits > 0x7fffffffLL || ngroups > 0x7fffffffLL)
+    return set_error(ZMPO_ERR_ARG, "dotgemm: too many units");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == ZMPO_F64)
+    return launch_dot<double>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+  if (dtype == ZMPO_F32)
+    return launch_dot<float>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+  return set_error(ZMPO_ERR_ARG, "dotgemm: unsupported dtype");
 }
 
 int zmpo_skinnygemm(int dtype, int rb, const zmpo_task* tasks, const zmpo_unit* units,

@@ -110,11 +110,31 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
             dmrg.nsigma += 1
             dmrg.sigma_flops += prog.flops
 
+        def hvec64(x, out):
+            prog.apply(dmrg.fhop, fL, fR, x, out, 0)
+            dmrg.nsigma64 = getattr(dmrg, 'nsigma64', 0) + 1
+            dmrg.sigma_flops += prog.flops
+
         solver.HVec = hvec
         v0 = None
         if dmrg.psi0 is not None:
             v0 = np.ascontiguousarray(dmrg.psi0[:neig][:, prjmap])
+        if dtype:
+            # low-precision sigma: converge to residual 1e-4 (what FP32 sigma
+            # resolves with margin), then
+            # restart Davidson from that solution with FP64 sigma (residual
+            # correction): the reported eigenvalue is an FP64 Ritz value
+            solver.crit_vec = max(dmrg.crit_vec, getattr(dmrg, 'crit_vec_lo', 1e-4))
+            solver.crit_e = max(dmrg.crit_e, getattr(dmrg, 'crit_e_lo', 1e-7))
+            solver.herm_tol = 1e-5
         eig, vec, nmvp = solver.solve_iter(v0)
+        if dtype:
+            fine = EigenSolver()
+            fine.crit_e, fine.crit_vec = dmrg.crit_e, dmrg.crit_vec
+            fine.ndim, fine.neig, fine.diag = ndim0, neig, solver.diag
+            fine.HVec = hvec64
+            eig, vec, nmvp64 = fine.solve_iter(dv.download(vec).reshape(neig, ndim0))
+            nmvp += nmvp64
         vec_h = dv.download(vec)
         civec = np.zeros((neig, dmrg.dims[3]))
         civec[:, prjmap] = vec_h

@@ -36,6 +36,7 @@ DEFAULT_BUDGET = 32 * 2 ** 30     # bytes of intermediates per op chunk
 FUSED = os.environ.get('ZMPO_FUSED', '1') != '0'
 CHAIN_RB = int(os.environ.get('ZMPO_CHAIN_RB', '1'))   # row tile of the chained launch: 16*rb
 SIGMA_STAGE1 = os.environ.get('ZMPO_STAGE1', 'shaped')   # grouped-GEMM kernel of L x psi
+SIGMA_STAGE4 = os.environ.get('ZMPO_STAGE4', 'dot')      # grouped-GEMM kernel of top3 x R
 # tile shapes (qtensor.SHAPES ids) of the L x psi and x R launches; '' = cost model
 SIGMA_SHAPES = [os.environ.get('ZMPO_SHAPE_S1', ''), os.environ.get('ZMPO_SHAPE_S4', '')]
 
@@ -240,7 +241,8 @@ class SigmaProgram:
                 progs.append(Program(t, pr, prefer=SIGMA_STAGE1 if k == 0 else None))
             t, pr = _merge_shared(parts[3], qtmp.nblks_allowed) if parts[3] else _concat([])
             # sigma blocks with the same right sector read the same R blocks
-            progs.append(Program(t, pr, reuse_key=qtmp.blk_isyms[:, 3] if len(t) else None))
+            progs.append(Program(t, pr, reuse_key=qtmp.blk_isyms[:, 3] if len(t) else None,
+                                 prefer=SIGMA_STAGE4))
             for k, sh in ((0, SIGMA_SHAPES[0]), (3, SIGMA_SHAPES[1])):
                 if sh:
                     progs[k].shape = int(sh)

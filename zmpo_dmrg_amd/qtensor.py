@@ -697,7 +697,7 @@ class Program:
             u = self.skinny_units(SKINNY_RB)
             self._sched[kernel] = {'units': dv.upload_bytes(u), 'nunits': len(u), 'rb': SKINNY_RB}
         if kernel not in self._sched:
-            tile = None if kernel == 'wave' else SHAPES[self.shape]
+            tile = None if kernel == 'wave' else (32, 32) if kernel == 'dot' else SHAPES[self.shape]
             units, groups, nslots = make_schedule(self.tasks_host, self.pairs_host, self.rb,
                                                   reuse_key=self.reuse_key, tile=tile)
             size = self.rb * 256 if kernel == 'wave' else tile[0] * tile[1]
@@ -756,6 +756,11 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None)
         return
     sc = prog.schedule(kernel)
     work = _workspace(sc['nslots'] * sc['slot_size'], dtype) if sc.get('nslots') else None
+    if kernel == 'dot':
+        _lib.check(L.zmpo_dotgemm(dtype, dv.ptr(tasks), dv.ptr(sc['units']), sc['nunits'],
+                                  dv.ptr(sc['groups']), sc['ngroups'], dv.ptr(pairs), dv.ptr(A),
+                                  dv.ptr(B), dv.ptr(C), dv.ptr(work), beta, st))
+        return
     if kernel == 'skinny':
         _lib.check(L.zmpo_skinnygemm(dtype, sc['rb'], dv.ptr(tasks), dv.ptr(sc['units']),
                                      sc['nunits'], dv.ptr(pairs), dv.ptr(A), dv.ptr(B), dv.ptr(C),
@@ -1023,8 +1028,10 @@ class ChainProgram:
                 continue
             us = self.units_host[sel]
             gsel = us['group']
-            # the widest class reads A from global memory so that its LDS
-            # (the intermediate only) still allows several workgroups per CU
+            # A blocks read from global memory (default: the LDS then holds
+            # only the intermediate, so more workgroups fit per CU; measured
+            # faster at K=36, D=1000) or staged in LDS; 'auto' stages them
+            # for every class but the widest
             atile = {'1': 1, '0': 0}.get(CHAIN_ATILE, int(k < len(CHAIN_CLASSES)))
             self.launches.append({'units': us, 'kmax': int(max(1, g['kcols'][gsel].max())),
                                   'nmax': int(max(1, g['ncols'][gsel].max())), 'atile': atile})
