@@ -58,6 +58,11 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ZmpoError('libzmpo.so not built at %s: run __graft_entry__.build()' % LIB_PATH)
+    # Bring up torch's HIP runtime before libzmpo pulls in the system one:
+    # the package shares device memory and streams with torch, and a process
+    # that loaded libzmpo first has seen its kernel launches report no device.
+    from . import device as _dv
+    _dv.torch().cuda.is_available()
     L = ctypes.CDLL(LIB_PATH)
     vp, i64, i32, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
     L.zmpo_last_error.restype = ctypes.c_char_p

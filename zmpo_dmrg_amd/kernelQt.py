@@ -586,6 +586,10 @@ def renormQt_H00(dmrg, isite, ncsite, flst, flstN, site, status):
     else:
         env_in = flst[0][1]
         idx = isite + ncsite - 1
-    new, _ = renorm(dmrg.fhop, idx, site, env_in, status, getattr(dmrg, 'dtype', 0))
+    # environments stay FP64 in every precision form: the FP64 residual
+    # correction of a low-precision Davidson solve needs an H_eff that is
+    # symmetric and accurate to FP64 (FP32-rounded environments leave a
+    # ~1e-7 relative asymmetry that bounds the corrected energy)
+    new, _ = renorm(dmrg.fhop, idx, site, env_in, status, getattr(dmrg, 'renorm_dtype', 0))
     flstN[0][0] = new
     return 0
