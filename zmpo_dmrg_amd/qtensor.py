@@ -917,14 +917,28 @@ def chain_structure(first, second, mid_key, out_key, mid_off, b1_map, b2_map):
     groups['kcols'] = kcols
     groups['ab_begin'] = ab[:-1]
     groups['ab_end'] = ab[1:]
-    # every block and output task of a group splits its rows the same way
+    # every block and output task of a group splits its rows the same way:
+    # 2-level maps (mlo < M) must agree; single-level ones (mlo >= M, e.g. a
+    # bra sector of dimension 1) are rewritten to that split, which leaves
+    # their offsets unchanged ((m / s) * s * lo + (m % s) * lo = m * lo)
     rsplit = np.zeros(len(keys), np.int64)
     for gi in range(len(keys)):
-        ms = set(blocks['a_mlo'][ab[gi]:ab[gi + 1]].tolist()) if len(blocks) else set()
-        ms |= set(tasks['c_mlo'][groups['t2_begin'][gi]:groups['t2_end'][gi]].tolist())
+        Mg_ = int(Mg[gi])
+        bsl = slice(ab[gi], ab[gi + 1])
+        tsl = slice(groups['t2_begin'][gi], groups['t2_end'][gi])
+        ms = set(int(x) for x in blocks['a_mlo'][bsl] if x < Mg_) if len(blocks) else set()
+        ms |= set(int(x) for x in tasks['c_mlo'][tsl] if x < Mg_)
         if len(ms) > 1:
             raise NotImplementedError('chained plans: row maps differ inside a row group')
-        rsplit[gi] = ms.pop() if ms else 1
+        rs = ms.pop() if ms else max(Mg_, 1)
+        rsplit[gi] = rs
+        if len(blocks):
+            one = blocks['a_mlo'][bsl] >= Mg_
+            blocks['a_smhi'][bsl] = np.where(one, rs * blocks['a_smlo'][bsl], blocks['a_smhi'][bsl])
+            blocks['a_mlo'][bsl] = np.where(one, rs, blocks['a_mlo'][bsl])
+        one = tasks['c_mlo'][tsl] >= Mg_
+        tasks['c_smhi'][tsl] = np.where(one, rs * tasks['c_smlo'][tsl], tasks['c_smhi'][tsl])
+        tasks['c_mlo'][tsl] = np.where(one, rs, tasks['c_mlo'][tsl])
     groups['rsplit'] = rsplit
     if len(blocks):
         r = np.maximum(Mg[np.repeat(np.arange(len(keys)), np.diff(ab))], 1) - 1
