@@ -51,6 +51,30 @@ def test_sigma_and_diag_on_device_match_oracle(gpu, case):
         assert np.abs(dv.download(dg) - case.oracle_diag(isite)).max() < 1e-12
 
 
+@pytest.mark.parametrize('overlap', [True, False])
+def test_sigma_chunked_two_streams_matches_oracle(gpu, case, overlap, monkeypatch):
+    """Three op chunks: L x psi of the next chunk on a side stream beside the
+    chained launch of the current one (double-buffered top1), or all on one
+    stream; sigma equals the oracle either way, repeated applications agree."""
+    monkeypatch.setattr(kq, 'SIGMA_CHUNKS', 3)
+    monkeypatch.setattr(kq, 'OVERLAP', overlap)
+    for isite in (1, case.K - 3):
+        qkey, oq, pt, _ = case.templates(isite)
+        v = np.random.default_rng(isite + 7).standard_normal(oq.size_allowed)
+        want = case.oracle_sigma(isite, v)
+        prog = kq.SigmaProgram(case.mpo, isite, pt, qkey, case.L[isite - 1], case.R[isite + 2],
+                               case.mpo.hpwts)
+        assert len(prog.chunks) >= 2
+        sig = dv.empty(pt.size_allowed, 0)
+        vd = dv.upload(v, 0)
+        outs = []
+        for _ in range(3):
+            prog.apply(case.mpo, case.L[isite - 1], case.R[isite + 2], vd, sig)
+            outs.append(dv.download(sig).copy())
+        assert np.abs(outs[0] - want).max() < 1e-12 * max(1.0, np.abs(want).max()), isite
+        assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
 def test_sigma_fp32_within_tolerance(gpu, case):
     isite = 1
     qkey, oq, pt, _ = case.templates(isite)

@@ -35,6 +35,11 @@ DEFAULT_BUDGET = 32 * 2 ** 30     # bytes of intermediates per op chunk
 # sigma stages 2-3 as one chained launch (intermediate top2 kept in LDS)
 FUSED = os.environ.get('ZMPO_FUSED', '1') != '0'
 CHAIN_RB = int(os.environ.get('ZMPO_CHAIN_RB', '1'))   # row tile of the chained launch: 16*rb
+# sigma op chunks: at least this many (1: as the budget allows); with two or
+# more, L x psi of chunk c+1 runs on a second stream beside the chained
+# launch of chunk c (both are latency-bound, so they share the GPU well)
+SIGMA_CHUNKS = int(os.environ.get('ZMPO_CHUNKS', '1'))
+OVERLAP = os.environ.get('ZMPO_OVERLAP', '1') != '0'
 SIGMA_STAGE1 = os.environ.get('ZMPO_STAGE1', 'shaped')   # grouped-GEMM kernel of L x psi
 SIGMA_STAGE4 = os.environ.get('ZMPO_STAGE4', 'dot')      # grouped-GEMM kernel of top3 x R
 # tile shapes (qtensor.SHAPES ids) of the L x psi and x R launches; '' = cost model
@@ -195,6 +200,8 @@ class SigmaProgram:
         self.case_of = [case(mpo.opers[p], isite, 2) for p in range(nops)]
         kept = (4, 6) if self.fused else (4, 5, 6)
         sz = [sum(plans[c][k].size_allowed for k in kept) * 8 for c in self.case_of]
+        if SIGMA_CHUNKS > 1:
+            budget = min(budget, max(1, sum(sz) // SIGMA_CHUNKS + 1))
         self.chunks = _chunks(sz, budget)
         self.stages = []
         self.chains = []
@@ -277,6 +284,9 @@ class SigmaProgram:
             return sigma
         if fused and not self.fused:
             raise ValueError('SigmaProgram built without the chained stages (fused=False)')
+        if fused and OVERLAP and len(self.stages) > 1 and not isinstance(psi, np.ndarray) \
+                and getattr(psi, 'is_cuda', False):
+            return self._apply_overlapped(mpo, L, R, psi, sigma, dtype, t1, t3)
         first = True
         for st, chain in zip(self.stages, self.chains):
             run_program(st[0], L, psi, t1, dtype)
@@ -289,6 +299,43 @@ class SigmaProgram:
             if st[3].ntasks:
                 run_program(st[3], t3, R, sigma, dtype, beta=0 if first else 1)
                 first = False
+        return sigma
+
+    def _apply_overlapped(self, mpo, L, R, psi, sigma, dtype, t1, t3):
+        """Chunked sigma with L x psi of chunk c+1 on a side stream while the
+        chained launch and x R of chunk c run on the current stream; top1 is
+        double-buffered, x R stays in chunk order (fixed summation order)."""
+        torch = dv.torch()
+        main = torch.cuda.current_stream()
+        if getattr(self, '_side', None) is None:
+            self._side = torch.cuda.Stream()   # default priority
+        side = self._side
+        key = ('t1b', dtype)
+        if key not in self._buffers:
+            self._buffers[key] = dv.empty(self.inter[0], dtype)
+        bufs = [t1, self._buffers[key]]
+        free = [None, None]
+        Pi, Pj = mpo.padded_buffer(self.isite, dtype), mpo.padded_buffer(self.isite + 1, dtype)
+        side.wait_stream(main)   # psi and the buffers are ready
+        first = True
+        for c, (st, chain) in enumerate(zip(self.stages, self.chains)):
+            buf = bufs[c % 2]
+            if free[c % 2] is not None:
+                side.wait_event(free[c % 2])   # chunk c-2 is done reading buf
+            run_program(st[0], L, psi, buf, dtype, stream=side.cuda_stream)
+            ready = torch.cuda.Event()
+            ready.record(side)
+            main.wait_event(ready)
+            run_chain(chain, buf, Pi, Pj, t3, dtype, stream=main.cuda_stream)
+            done = torch.cuda.Event()
+            done.record(main)
+            free[c % 2] = done
+            if st[3].ntasks:
+                run_program(st[3], t3, R, sigma, dtype, beta=0 if first else 1,
+                            stream=main.cuda_stream)
+                first = False
+        if first:
+            sigma.zero_()
         return sigma
 
 

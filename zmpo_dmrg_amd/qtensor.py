@@ -623,25 +623,67 @@ class Program:
     def __init__(self, tasks, pairs, reuse_key=None, prefer=None):
         self.reuse_key = reuse_key
         self.prefer = prefer   # kernel to use when eligible (None: the library default)
-        tasks, pairs = collapse_maps(tasks, pairs)
         self.ntasks = len(tasks)
         self.npairs = len(pairs)
-        self.ntiles = finish_tasks(tasks) if self.ntasks else 0
-        self.tasks_host, self.pairs_host = tasks, pairs
+        self._raw = (tasks, pairs)
+        self._host = None
         self._dev = None
         self._sched = None
+        self._rb = self._shape = self._small = None
         rows = np.repeat(np.arange(self.ntasks), (tasks['p_end'] - tasks['p_begin']).astype(np.int64))
         self.flops = int(np.sum(2 * tasks['M'][rows].astype(np.int64) * tasks['N'][rows]
                                 * pairs['K'].astype(np.int64))) if self.npairs else 0
-        self.rb = choose_rb(tasks) if self.ntasks else 1
-        self.shape = choose_shape(tasks, pairs) if self.ntasks else 0
         self.kt = None    # stream kernel k-tile (None: library default)
         self.upw = None   # stream kernel units per workgroup (None: by launch size)
-        # the stream kernel keeps in-block row offsets in 32 bits
-        self.small_blocks = bool(self.npairs == 0 or max(
-            _max_row_offset(pairs['a_mlo'], pairs['a_smhi'], pairs['a_smlo'], tasks['M'][rows]),
-            _max_row_offset(pairs['b_nlo'], pairs['b_snhi'], pairs['b_snlo'], tasks['N'][rows]))
-            < 2 ** 31 - 1)
+
+    # The device-ready arrays and the schedule choices are derived on first
+    # use (a sigma plan builds programs that only the unfused path launches).
+    def _prepare(self):
+        if self._host is None:
+            t, p = collapse_maps(*self._raw)
+            self.ntiles = finish_tasks(t) if self.ntasks else 0
+            self._host = (t, p)
+            self._raw = None
+        return self._host
+
+    @property
+    def tasks_host(self):
+        return self._prepare()[0]
+
+    @property
+    def pairs_host(self):
+        return self._prepare()[1]
+
+    @property
+    def rb(self):
+        if self._rb is None:
+            self._rb = choose_rb(self.tasks_host) if self.ntasks else 1
+        return self._rb
+
+    @rb.setter
+    def rb(self, v):
+        self._rb = v
+
+    @property
+    def shape(self):
+        if self._shape is None:
+            self._shape = choose_shape(self.tasks_host, self.pairs_host) if self.ntasks else 0
+        return self._shape
+
+    @shape.setter
+    def shape(self, v):
+        self._shape = v
+
+    @property
+    def small_blocks(self):
+        """Whether in-block row offsets fit 32 bits (the stream kernel's)."""
+        if self._small is None:
+            t, p = self.tasks_host, self.pairs_host
+            rows = np.repeat(np.arange(self.ntasks), (t['p_end'] - t['p_begin']).astype(np.int64))
+            self._small = bool(self.npairs == 0 or max(
+                _max_row_offset(p['a_mlo'], p['a_smhi'], p['a_smlo'], t['M'][rows]),
+                _max_row_offset(p['b_nlo'], p['b_snhi'], p['b_snlo'], t['N'][rows])) < 2 ** 31 - 1)
+        return self._small
 
     def skinny_ok(self):
         """Whether zmpo_skinnygemm applies: K-contiguous single-level A rows."""
@@ -783,6 +825,27 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None)
                                      dv.ptr(work), beta, st))
 
 
+def _map_lookup(offs, bmap):
+    """(new offset, row stride) of every B block offset through a padded-copy map."""
+    if len(offs) == 0:
+        return np.zeros((0, 2), np.int64)
+    ks = np.fromiter(bmap.keys(), np.int64, len(bmap))
+    vs = np.array(list(bmap.values()), np.int64).reshape(-1, 2)
+    o = np.argsort(ks)
+    ks, vs = ks[o], vs[o]
+    i = np.searchsorted(ks, offs)
+    if np.any(i >= len(ks)) or np.any(ks[np.minimum(i, len(ks) - 1)] != offs):
+        raise RuntimeError('chained plans: B block missing from the padded MPO map')
+    return vs[i]
+
+
+def _segments(sizes):
+    """Exclusive prefix sum and, per element, its segment's start."""
+    sizes = np.asarray(sizes, np.int64)
+    starts = np.cumsum(sizes) - sizes
+    return starts, np.repeat(starts, sizes)
+
+
 def chain_structure(first, second, mid_key, out_key, mid_off, b1_map, b2_map):
     """Row-group structure of two chained plans (one operator case).
 
@@ -796,115 +859,116 @@ def chain_structure(first, second, mid_key, out_key, mid_off, b1_map, b2_map):
     reserved in multiples of 4 (the padding columns hold zeros).  Returns
     (groups, tasks, pairs, blocks) for zmpo_chaingemm with offsets relative
     to each operand's base; groups without output blocks are dropped (their
-    intermediate is never read).
+    intermediate is never read).  Vectorised (numpy only).
     """
     t1, p1 = first['tasks'], first['pairs']
     t2, p2 = second['tasks'], second['pairs']
     mid_key = np.asarray(mid_key, np.int64)
     out_key = np.asarray(out_key, np.int64)
+    mid_off = np.asarray(mid_off, np.int64)
     keys = np.unique(out_key)
-    col = np.zeros(len(t1), np.int64)
+    ng = len(keys)
     grp_of_mid = np.full(len(t1), -1, np.int64)
     gi = np.searchsorted(keys, mid_key)
-    ok = (gi < len(keys)) & (keys[np.minimum(gi, len(keys) - 1)] == mid_key)
+    ok = (gi < ng) & (keys[np.minimum(gi, max(ng - 1, 0))] == mid_key) if ng else np.zeros(len(t1), bool)
     grp_of_mid[ok] = gi[ok]
-    # intermediate column offsets: cumulative widths inside each group
+    # intermediate column offsets: cumulative widths (multiples of 4) per group
     order1 = np.lexsort((np.arange(len(t1)), grp_of_mid))
     order1 = order1[grp_of_mid[order1] >= 0]
     g1 = grp_of_mid[order1]
+    n1 = np.bincount(g1, minlength=ng).astype(np.int64)
     w1 = (t1['N'][order1].astype(np.int64) + 3) // 4 * 4
+    col = np.zeros(len(t1), np.int64)
     csum = np.cumsum(w1) - w1
-    start = np.concatenate([[0], np.flatnonzero(np.diff(g1)) + 1]) if len(g1) else np.zeros(0, np.int64)
-    first_of = np.repeat(csum[start], np.diff(np.append(start, len(g1)))) if len(g1) else csum
-    col[order1] = csum - first_of
-    n1 = np.bincount(g1, minlength=len(keys))
-    ncols = np.bincount(g1, weights=w1, minlength=len(keys)).astype(np.int64)
+    _, gstart1 = _segments(n1)
+    col[order1] = csum - csum[gstart1] if len(g1) else csum
+    ncols = np.bincount(g1, weights=w1, minlength=ng).astype(np.int64)
     grp_of_out = np.searchsorted(keys, out_key)
     order2 = np.lexsort((np.arange(len(t2)), grp_of_out))
-    n2 = np.bincount(grp_of_out, minlength=len(keys))
+    g2 = grp_of_out[order2]
+    n2 = np.bincount(g2, minlength=ng).astype(np.int64)
     # the intermediate block each second-stage pair reads
-    mid_idx = {int(o): j for j, o in enumerate(mid_off)}
-    pa = np.array([mid_idx[int(o)] for o in p2['a_off']], dtype=np.int64) if len(p2) \
-        else np.zeros(0, np.int64)
-    rows2 = np.repeat(np.arange(len(t2)), (t2['p_end'] - t2['p_begin']).astype(np.int64))
+    pa = np.searchsorted(mid_off, p2['a_off'].astype(np.int64))
+    if len(pa) and (np.any(pa >= len(mid_off)) or
+                    np.any(mid_off[np.minimum(pa, len(mid_off) - 1)] != p2['a_off'])):
+        raise RuntimeError('chained plans: a second-stage pair reads no intermediate block')
+    rows2 = pair_tasks(t2, len(p2))
     if len(pa) and np.any(grp_of_mid[pa] != grp_of_out[rows2]):
         raise RuntimeError('chained plans: a second-stage pair reads another row group')
-    # tasks: per group its first-stage tasks, then its second-stage tasks
-    tb1 = np.concatenate([[0], np.cumsum(n1 + n2)[:-1]]).astype(np.int64)
-    tasks = np.zeros(int((n1 + n2).sum()), _lib.TASK_DTYPE)
-    pairs_parts = []
-    pcount = 0
-    new_t1 = np.empty(len(order1), np.int64)
-    pos1 = tb1[g1] + (np.arange(len(g1)) - (np.repeat(start, np.diff(np.append(start, len(g1))))
-                                              if len(g1) else 0))
-    new_t1[:] = pos1
-    g2 = grp_of_out[order2]
-    s2 = np.concatenate([[0], np.flatnonzero(np.diff(g2)) + 1]) if len(g2) else np.zeros(0, np.int64)
-    pos2 = tb1[g2] + n1[g2] + (np.arange(len(g2)) - np.repeat(s2, np.diff(np.append(s2, len(g2))))
-                               if len(g2) else 0)
-    src = np.empty(len(tasks), np.int64)
-    stage = np.zeros(len(tasks), np.int8)
-    src[pos1] = order1
-    src[pos2] = order2
-    stage[pos2] = 1
-    p1 = p1.copy()
-    p2 = p2.copy()
-    if len(pa):
-        p2['a_off'] = col[pa]
-    for pp, bmap in ((p1, b1_map), (p2, b2_map)):
-        if len(pp) == 0:
-            continue
-        if np.any(pp['b_nlo'] < np.repeat(t1['N'] if pp is p1 else t2['N'],
-                                          ((t1 if pp is p1 else t2)['p_end'] -
-                                           (t1 if pp is p1 else t2)['p_begin']).astype(np.int64))):
-            raise NotImplementedError('chained plans: B column map is not single-level')
-        nb = np.array([bmap[int(o)] for o in pp['b_off']], dtype=np.int64).reshape(-1, 2)
-        pp['b_off'] = nb[:, 0]
-        pp['b_sk'] = nb[:, 1]
-        pp['b_snlo'] = 1
-        pp['b_snhi'] = 0
-        pp['b_nlo'] = 1 << 30
+    rows1 = pair_tasks(t1, len(p1))
+    if len(p1) and np.any(p1['b_nlo'] < t1['N'][rows1]) or \
+            len(p2) and np.any(p2['b_nlo'] < t2['N'][rows2]):
+        raise NotImplementedError('chained plans: B column map is not single-level')
     if np.any(p1['scale'] != 1.0) or np.any(p2['scale'] != 1.0):
         raise NotImplementedError('chained plans: pair scales must be 1')
-    for ti in range(len(tasks)):
-        j = int(src[ti])
-        if stage[ti] == 0:
-            tk = t1[j]
-            pp = p1[int(tk['p_begin']):int(tk['p_end'])]
-        else:
-            tk = t2[j]
-            pp = p2[int(tk['p_begin']):int(tk['p_end'])]
-        tasks[ti] = tk
-        if stage[ti] == 0:
-            tasks['c_off'][ti] = col[j]
-        tasks['p_begin'][ti] = pcount
-        pcount += len(pp)
-        tasks['p_end'][ti] = pcount
-        pairs_parts.append(pp)
-    pairs = np.concatenate(pairs_parts) if pairs_parts else np.zeros(0, _lib.PAIR_DTYPE)
-    # distinct first-stage A blocks per group, staged as A-tile columns
-    blocks = []
-    kcols = np.zeros(len(keys), np.int64)
-    ab = np.zeros(len(keys) + 1, np.int64)
-    for gi in range(len(keys)):
-        ab[gi] = len(blocks)
-        lo = tasks['p_begin'][tb1[gi]] if n1[gi] else 0
-        hi = tasks['p_end'][tb1[gi] + n1[gi] - 1] if n1[gi] else 0
-        seen = {}
-        for q in range(lo, hi):
-            ao = int(pairs['a_off'][q])
-            if ao not in seen:
-                seen[ao] = (int(kcols[gi]), len(blocks))
-                blocks.append((ao, int(pairs['a_smhi'][q]), int(pairs['a_smlo'][q]),
-                               int(pairs['a_sk'][q]), int(pairs['K'][q]), int(kcols[gi]),
-                               int(pairs['a_mlo'][q]), 0))
-                kcols[gi] += (int(pairs['K'][q]) + 3) // 4 * 4
-            pairs['a_off'][q] = seen[ao][0]
-            pairs['pad'][q] = seen[ao][1]    # the A block, for reads from global memory
-    ab[len(keys)] = len(blocks)
-    blocks = np.array(blocks, dtype=_lib.CBLOCK_DTYPE) if blocks else np.zeros(0, _lib.CBLOCK_DTYPE)
-    groups = np.zeros(len(keys), _lib.CGROUP_DTYPE)
-    Mg = np.zeros(len(keys), np.int64)
+    # pairs with B re-pointed at the padded MPO copies; second-stage A = columns
+    P = np.concatenate([p1, p2])
+    nb = np.concatenate([_map_lookup(p1['b_off'].astype(np.int64), b1_map),
+                         _map_lookup(p2['b_off'].astype(np.int64), b2_map)])
+    if len(P):
+        P['b_off'] = nb[:, 0]
+        P['b_sk'] = nb[:, 1]
+        P['b_snlo'] = 1
+        P['b_snhi'] = 0
+        P['b_nlo'] = 1 << 30
+        P['a_off'][len(p1):] = col[pa]
+    # tasks: per group its first-stage tasks, then its second-stage tasks
+    tb1 = np.cumsum(n1 + n2) - (n1 + n2)
+    ntask = int((n1 + n2).sum())
+    tasks = np.zeros(ntask, _lib.TASK_DTYPE)
+    pos1 = tb1[g1] + (np.arange(len(g1)) - gstart1) if len(g1) else np.zeros(0, np.int64)
+    _, gstart2 = _segments(n2)
+    pos2 = tb1[g2] + n1[g2] + (np.arange(len(g2)) - gstart2) if len(g2) else np.zeros(0, np.int64)
+    tasks[pos1] = t1[order1]
+    tasks[pos2] = t2[order2]
+    tasks['c_off'][pos1] = col[order1]
+    src_pb = np.zeros(ntask, np.int64)
+    src_pb[pos1] = t1['p_begin'][order1]
+    src_pb[pos2] = t2['p_begin'][order2].astype(np.int64) + len(p1)
+    cnt = (tasks['p_end'] - tasks['p_begin']).astype(np.int64)
+    new_pb, seg = _segments(cnt)
+    gather = np.repeat(src_pb, cnt) + (np.arange(int(cnt.sum())) - seg)
+    pairs = P[gather]
+    tasks['p_begin'] = new_pb
+    tasks['p_end'] = new_pb + cnt
+    stage_of_task = np.zeros(ntask, np.int64)
+    stage_of_task[pos2] = 1
+    group_of_task = np.zeros(ntask, np.int64)
+    group_of_task[pos1] = g1
+    group_of_task[pos2] = g2
+    prow = np.repeat(np.arange(ntask), cnt)
+    # distinct first-stage A blocks (each top1 block belongs to one group),
+    # ordered by group then first use, at A-tile columns of width ceil4(K)
+    s1 = np.flatnonzero(stage_of_task[prow] == 0)
+    if len(s1):
+        ao = pairs['a_off'][s1].astype(np.int64)
+        uniq, first_i, inv = np.unique(ao, return_index=True, return_inverse=True)
+        bgrp = group_of_task[prow[s1[first_i]]]
+        border = np.lexsort((first_i, bgrp))
+        rank = np.empty(len(uniq), np.int64)
+        rank[border] = np.arange(len(uniq))
+        q = s1[first_i[border]]
+        blocks = np.zeros(len(uniq), _lib.CBLOCK_DTYPE)
+        blocks['a_off'] = uniq[border]
+        for f in ('a_smhi', 'a_smlo', 'a_sk', 'K', 'a_mlo'):
+            blocks[f] = pairs[f][q]
+        bg = bgrp[border]
+        kw = (blocks['K'].astype(np.int64) + 3) // 4 * 4
+        nblk = np.bincount(bg, minlength=ng).astype(np.int64)
+        bstart, bseg = _segments(nblk)
+        kc = np.cumsum(kw) - kw
+        blocks['col'] = kc - kc[bseg]
+        kcols = np.bincount(bg, weights=kw, minlength=ng).astype(np.int64)
+        pairs['a_off'][s1] = blocks['col'][rank[inv]]
+        pairs['pad'][s1] = rank[inv]
+        ab = np.append(bstart, len(uniq))
+    else:
+        blocks = np.zeros(0, _lib.CBLOCK_DTYPE)
+        kcols = np.zeros(ng, np.int64)
+        ab = np.zeros(ng + 1, np.int64)
+        bg = np.zeros(0, np.int64)
+    groups = np.zeros(ng, _lib.CGROUP_DTYPE)
+    Mg = np.zeros(ng, np.int64)
     Mg[g2] = t2['M'][order2]
     if len(g1) and np.any(t1['M'][order1] != Mg[g1]):
         raise RuntimeError('chained plans: blocks of one row group differ in rows')
@@ -921,41 +985,38 @@ def chain_structure(first, second, mid_key, out_key, mid_off, b1_map, b2_map):
     # 2-level maps (mlo < M) must agree; single-level ones (mlo >= M, e.g. a
     # bra sector of dimension 1) are rewritten to that split, which leaves
     # their offsets unchanged ((m / s) * s * lo + (m % s) * lo = m * lo)
-    rsplit = np.zeros(len(keys), np.int64)
-    for gi in range(len(keys)):
-        Mg_ = int(Mg[gi])
-        bsl = slice(ab[gi], ab[gi + 1])
-        tsl = slice(groups['t2_begin'][gi], groups['t2_end'][gi])
-        ms = set(int(x) for x in blocks['a_mlo'][bsl] if x < Mg_) if len(blocks) else set()
-        ms |= set(int(x) for x in tasks['c_mlo'][tsl] if x < Mg_)
-        if len(ms) > 1:
-            raise NotImplementedError('chained plans: row maps differ inside a row group')
-        rs = ms.pop() if ms else max(Mg_, 1)
-        rsplit[gi] = rs
-        if len(blocks):
-            one = blocks['a_mlo'][bsl] >= Mg_
-            blocks['a_smhi'][bsl] = np.where(one, rs * blocks['a_smlo'][bsl], blocks['a_smhi'][bsl])
-            blocks['a_mlo'][bsl] = np.where(one, rs, blocks['a_mlo'][bsl])
-        one = tasks['c_mlo'][tsl] >= Mg_
-        tasks['c_smhi'][tsl] = np.where(one, rs * tasks['c_smlo'][tsl], tasks['c_smhi'][tsl])
-        tasks['c_mlo'][tsl] = np.where(one, rs, tasks['c_mlo'][tsl])
+    t2i = np.flatnonzero(stage_of_task == 1)
+    vals = np.concatenate([blocks['a_mlo'].astype(np.int64), tasks['c_mlo'][t2i].astype(np.int64)])
+    vgrp = np.concatenate([bg, group_of_task[t2i]])
+    two = vals < Mg[vgrp] if len(vals) else np.zeros(0, bool)
+    lo_ = np.full(ng, np.iinfo(np.int64).max)
+    hi_ = np.full(ng, -1, np.int64)
+    np.minimum.at(lo_, vgrp[two], vals[two])
+    np.maximum.at(hi_, vgrp[two], vals[two])
+    if np.any((hi_ >= 0) & (hi_ != lo_)):
+        raise NotImplementedError('chained plans: row maps differ inside a row group')
+    rsplit = np.where(hi_ >= 0, hi_, np.maximum(Mg, 1))
     groups['rsplit'] = rsplit
     if len(blocks):
-        r = np.maximum(Mg[np.repeat(np.arange(len(keys)), np.diff(ab))], 1) - 1
+        one = blocks['a_mlo'] >= Mg[bg]
+        blocks['a_smhi'] = np.where(one, rsplit[bg] * blocks['a_smlo'], blocks['a_smhi'])
+        blocks['a_mlo'] = np.where(one, rsplit[bg], blocks['a_mlo'])
+        r = np.maximum(Mg[bg], 1) - 1
         big = (r // blocks['a_mlo']) * blocks['a_smhi'] + np.minimum(r, blocks['a_mlo'] - 1) * \
             blocks['a_smlo'] + (blocks['K'].astype(np.int64) - 1) * blocks['a_sk']
         if big.max() >= 2 ** 31:
             raise NotImplementedError('chained plans: an A block exceeds 2^31 elements')
+    if len(t2i):
+        tg = group_of_task[t2i]
+        one = tasks['c_mlo'][t2i] >= Mg[tg]
+        tasks['c_smhi'][t2i] = np.where(one, rsplit[tg] * tasks['c_smlo'][t2i], tasks['c_smhi'][t2i])
+        tasks['c_mlo'][t2i] = np.where(one, rsplit[tg], tasks['c_mlo'][t2i])
     # inside each stage of each group, costly tasks first: the waves take
     # tasks round-robin, so this balances them (pair ranges are unchanged)
     kp = (pairs['K'].astype(np.int64) + 3) // 4
-    prow = np.repeat(np.arange(len(tasks)), (tasks['p_end'] - tasks['p_begin']).astype(np.int64))
-    cost = np.bincount(prow, weights=kp, minlength=len(tasks)) * ((tasks['N'] + 31) // 32)
-    for lo, hi in zip(np.concatenate([groups['t1_begin'], groups['t2_begin']]),
-                      np.concatenate([groups['t1_end'], groups['t2_end']])):
-        if hi - lo > 1:
-            o = lo + np.argsort(-cost[lo:hi], kind='stable')
-            tasks[lo:hi] = tasks[o]
+    cost = np.bincount(prow, weights=kp, minlength=ntask) * ((tasks['N'] + 31) // 32)
+    segid = 2 * group_of_task + stage_of_task
+    tasks = tasks[np.lexsort((np.arange(ntask), -cost, segid))]
     return groups, tasks, pairs, blocks
 
 
