@@ -242,6 +242,10 @@ def main():
     bound = 'hbm' if stage_fl[dom] / stage_by[dom] < PEAK[dtype] / HBM else 'mfma'
     kname = 'chained grouped GEMM (zmpo_chaingemm)' if (prog.fused and dom == 1) else \
         '%s grouped GEMM' % pq.KERNEL
+    # kernel dispatches behind one launch of the dominant stage (the chained
+    # launch issues one dispatch per LDS size class), so the per-dispatch
+    # average can be set beside rocprofv3's kernel statistics
+    ndisp = sum(len(ch.launches) for ch in prog.chains) if (prog.fused and dom == 1) else nlaunch
     roof = {'kernel': '%s, sigma launch %s' % (kname, names[dom]),
             'bound': bound,
             'achieved': ach_b / 1e9 if bound == 'hbm' else ach_f / 1e12,
@@ -253,6 +257,7 @@ def main():
             'flops_per_launch': stage_fl[dom] / nlaunch,
             'achieved_tflops': ach_f / 1e12, 'mfma_frac': ach_f / PEAK[dtype],
             'launch_ms': avg_ms, 'launches_per_step': nlaunch,
+            'dispatches_per_step': ndisp, 'dispatch_avg_ms': stage_ms[dom] / ndisp,
             'stage_names': names,
             'stage_ms': [round(x, 4) for x in stage_ms],
             'stage_gflop': [round(x / 1e9, 3) for x in stage_fl],

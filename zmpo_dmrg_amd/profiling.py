@@ -20,7 +20,7 @@ import json
 import sys
 
 KERNELS = ('shaped_gemm', 'shaped_reduce', 'chain_gemm', 'stream_gemm', 'wave_gemm', 'wave_reduce',
-           'skinny_gemm', 'blockgemm')
+           'skinny_gemm', 'dot_gemm', 'blockgemm')
 
 
 def _copy(gib):
