@@ -20,6 +20,9 @@ Pinning (what checks each module against something other than itself):
                               with the MPS (Hermiticity, diag == HDiag,
                               <psi|H|psi> == Davidson eigenvalue).
   davidson                 -- scipy.linalg.eigh on the explicit matrix.
+  fci.fci_rdms             -- E = sum h gamma + 1/2 sum (pr|qs) Gamma, tr gamma = N,
+                              and central differences of the FCI energy
+                              (Hellmann-Feynman) along h and symmetric eri.
   sweep                    -- exact FCI (oracle.fci) at K<=6 with D >= full
                               dimension, and the H6/STO-3G FCI energy recorded
                               in the reference (samps/example_conversion/

@@ -94,3 +94,28 @@ def mpo_dense(mpo_sites, K, nops):
             t = t.reshape(s[0], s[1] * s[2], s[3] * s[4])
         H += t[0]
     return H
+
+
+def fci_rdms(h, eri, nelec, sz=0.0):
+    """Ground state energy and spin-summed RDMs in the (N, Sz) sector:
+    gamma_pr = <E_pr>, Gamma_prqs = <E_pr E_qs> - delta_qr <E_ps>, so that
+    E = sum h gamma + 1/2 sum (pr|qs) Gamma (the Hamiltonian of fock_hamiltonian).
+    The quantities of properties/mpo_dmrg_propsItrf.py:204-263 (<E_pq>) and
+    mpo_dmrg_propsMPO.py:73-141 (<a+a+aa>), here from the exact vector."""
+    K = h.shape[0]
+    H, _ = fock_hamiltonian(h, eri)
+    a, _ = _annihilators(K)
+    idx = sector_states(K, nelec, sz)
+    w, v = np.linalg.eigh(H[idx][:, idx].toarray())
+    c = np.zeros(4 ** K)
+    c[idx] = v[:, 0]
+    V = np.zeros((K, K, 4 ** K))
+    for q in range(K):
+        for s in range(K):
+            Eqs = a[2 * q].T @ a[2 * s] + a[2 * q + 1].T @ a[2 * s + 1]
+            V[q, s] = Eqs @ c
+    gamma = np.einsum('d,qsd->qs', c, V)
+    # <E_pr E_qs> = <E_rp c, E_qs c> for a real vector
+    G = np.einsum('rpd,qsd->prqs', V, V)
+    G -= np.einsum('qr,ps->prqs', np.eye(K), gamma)
+    return float(w[0]), gamma, G

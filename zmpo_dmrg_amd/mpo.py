@@ -131,10 +131,10 @@ def _sort_key_order(q):
     return np.array(sorted(range(len(q)), key=lambda k: keys[k]))
 
 
-def site_tensor(p, nsite, isite, int1e, int2e):
-    """Block-sparse H_p on spatial site isite, legs [wl, wr, u, d], status [F,T,F,T]."""
-    hq, vqrs = int1e[p], int2e[p]
-    w = link(hfac_spin(p, nsite, 2 * isite, hq, vqrs), hfac_spin(p, nsite, 2 * isite + 1, hq, vqrs))
+def site_order(p, nsite, isite):
+    """Bond quantum numbers of H_p on spatial site isite and the row / column
+    orders (il, ir) that sort them (qtensor_opers.py:389-404): the stored
+    tensor is the dense link(...)[il][:, ir]."""
     isz = p % 2
     el, _ = _elem_q(p, 2 * isite)
     _, er = _elem_q(p, 2 * isite + 1)
@@ -142,7 +142,14 @@ def site_tensor(p, nsite, isite, int1e, int2e):
     _, wr = _w_q(nsite, 2 * isite + 1, isz)
     ql = wl + el[None, :]
     qr = wr + er[None, :]
-    il, ir = _sort_key_order(ql), _sort_key_order(qr)
+    return ql, qr, _sort_key_order(ql), _sort_key_order(qr)
+
+
+def site_tensor(p, nsite, isite, int1e, int2e):
+    """Block-sparse H_p on spatial site isite, legs [wl, wr, u, d], status [F,T,F,T]."""
+    hq, vqrs = int1e[p], int2e[p]
+    w = link(hfac_spin(p, nsite, 2 * isite, hq, vqrs), hfac_spin(p, nsite, 2 * isite + 1, hq, vqrs))
+    ql, qr, il, ir = site_order(p, nsite, isite)
     w = w[np.ix_(il, ir)]
     qt = QTensor([False, True, False, True])
     qp = qphys()
