@@ -261,6 +261,12 @@ class _CpuVecOps:
         o[:] = alpha * acc if beta == 0.0 else beta * o + alpha * acc
         return out
 
+    def gram(self, X, nx, Y, ny):
+        return np.array([self.dots(X, nx, Y[i]) for i in range(ny)])
+
+    def project_out(self, r, X, nx):
+        return self.lincomb(r, X, self.dots(X, nx, r), -1.0, 1.0)
+
     def norm(self, x):
         return float(np.linalg.norm(_np(x)))
 

@@ -424,9 +424,14 @@ __global__ __launch_bounds__(256) void shaped_gemm_kernel(
   constexpr int BM = TMv / 16, BN = TNv / 16;
   constexpr int NBLK = BM * BN;
   constexpr int PER = (NBLK + 3) / 4;
-  __shared__ T As[2][KT][TMv + 1];
-  __shared__ T Bs[2][KT][TNv + 1];
-  __shared__ T stage[TMv * TNv];
+  // the output staging tile reuses the operand buffers (the k loop ends with
+  // a barrier): less LDS per workgroup, more workgroups per CU
+  constexpr int NA = 2 * KT * (TMv + 1), NB = 2 * KT * (TNv + 1);
+  constexpr int NS = NA + NB > TMv * TNv ? NA + NB : TMv * TNv;
+  __shared__ T smem[NS];
+  T(*As)[KT][TMv + 1] = reinterpret_cast<T(*)[KT][TMv + 1]>(smem);
+  T(*Bs)[KT][TNv + 1] = reinterpret_cast<T(*)[KT][TNv + 1]>(smem + NA);
+  T* stage = smem;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = wave_id();
   const int lr = lane & 15, lk = lane >> 4;

@@ -185,14 +185,12 @@ class WGradProgram:
             parts = [[], [], []]
             for p in ch:
                 pl = plans[case_of[p]]
-                shift = [sum(len(x_[1]) for x_ in parts[k]) for k in range(3)]
-                parts[0].append(kq._instantiate(pl[0], envL.off[p], 0, offs[0], 1.0, shift[0]))
-                parts[1].append(kq._instantiate(pl[1], envR.off[p], 0, offs[1], 1.0, shift[1]))
-                parts[2].append(kq._instantiate(pl[2], offs[0], offs[1], self.out_off[p],
-                                                float(hpwts[p]), shift[2]))
+                parts[0].append((pl[0], envL.off[p], 0, offs[0], 1.0))
+                parts[1].append((pl[1], envR.off[p], 0, offs[1], 1.0))
+                parts[2].append((pl[2], offs[0], offs[1], self.out_off[p], float(hpwts[p])))
                 offs[0] += pl[3].size_allowed
                 offs[1] += pl[4].size_allowed
-            progs = [Program(*kq._concat(parts[k])) for k in range(3)]
+            progs = [Program(*kq._instantiate_many(parts[k])) for k in range(3)]
             self.flops += sum(pg.flops for pg in progs)
             self.stages.append(progs)
             self.inter = [max(a, b) for a, b in zip(self.inter, offs)]

@@ -41,9 +41,13 @@ CHAIN_RB = int(os.environ.get('ZMPO_CHAIN_RB', '1'))   # row tile of the chained
 SIGMA_CHUNKS = int(os.environ.get('ZMPO_CHUNKS', '1'))
 OVERLAP = os.environ.get('ZMPO_OVERLAP', '1') != '0'
 SIGMA_STAGE1 = os.environ.get('ZMPO_STAGE1', 'shaped')   # grouped-GEMM kernel of L x psi
-SIGMA_STAGE4 = os.environ.get('ZMPO_STAGE4', 'dot')      # grouped-GEMM kernel of top3 x R
-# tile shapes (qtensor.SHAPES ids) of the L x psi and x R launches; '' = cost model
-SIGMA_SHAPES = [os.environ.get('ZMPO_SHAPE_S1', ''), os.environ.get('ZMPO_SHAPE_S4', '')]
+SIGMA_STAGE4 = os.environ.get('ZMPO_STAGE4', 'shaped')   # grouped-GEMM kernel of top3 x R
+# tile shapes (qtensor.SHAPES ids) of the L x psi and x R launches ('' = cost
+# model).  32x32 for both: a sigma block's right dimension (~26 at D = 1000)
+# fits one tile width, and the small LDS footprint keeps 8 workgroups per CU
+# (measured at K = 36, D = 1000: L x psi 15.6 ms vs 16.6 for 64x16, x R 13.9
+# ms vs 17.5 for zmpo_dotgemm)
+SIGMA_SHAPES = [os.environ.get('ZMPO_SHAPE_S1', '2'), os.environ.get('ZMPO_SHAPE_S4', '2')]
 
 
 def case(porb, isite, ncsite=1):
@@ -104,46 +108,45 @@ def boundary_store(nops, nqnum=2):
     return EnvStore([qt] * nops, np.ones(nops), np.arange(nops))
 
 
-def _instantiate(lowered, a_base, b_base, c_base, scale=1.0, p_shift=0):
-    """Copy a case plan for one op with operand base offsets."""
-    t = lowered['tasks'].copy()
-    p = lowered['pairs'].copy()
-    p['a_off'] += a_base
-    p['b_off'] += b_base
-    p['scale'] = scale
-    t['c_off'] += c_base
-    t['p_begin'] += p_shift
-    t['p_end'] += p_shift
+def _instantiate_many(items):
+    """Case plans instantiated for a list of ops, concatenated: items =
+    [(lowered, a_base, b_base, c_base, scale)] adds the op's operand base
+    offsets to a copy of its case plan, sets the pair scale and shifts the
+    op's p_begin/p_end past the pairs of the ops before it."""
+    if not items:
+        return np.zeros(0, _lib.TASK_DTYPE), np.zeros(0, _lib.PAIR_DTYPE)
+    # concatenated as raw bytes (numpy copies structured arrays field by field)
+    t = np.concatenate([it[0]['tasks'].view(np.uint8) for it in items]).view(_lib.TASK_DTYPE)
+    p = np.concatenate([it[0]['pairs'].view(np.uint8) for it in items]).view(_lib.PAIR_DTYPE)
+    nt = np.array([len(it[0]['tasks']) for it in items], dtype=np.int64)
+    npr = np.array([len(it[0]['pairs']) for it in items], dtype=np.int64)
+    base = np.array([it[1:4] for it in items], dtype=np.int64)
+    p['a_off'] += np.repeat(base[:, 0], npr)
+    p['b_off'] += np.repeat(base[:, 1], npr)
+    p['scale'] = np.repeat(np.array([it[4] for it in items], dtype=np.float64), npr)
+    t['c_off'] += np.repeat(base[:, 2], nt)
+    shift = np.repeat(np.concatenate([[0], np.cumsum(npr)[:-1]]), nt)
+    t['p_begin'] += shift
+    t['p_end'] += shift
     return t, p
 
 
-def _concat(parts):
-    ts = [t for t, _ in parts]
-    ps = [p for _, p in parts]
-    if not ts:
-        return np.zeros(0, _lib.TASK_DTYPE), np.zeros(0, _lib.PAIR_DTYPE)
-    return np.concatenate(ts), np.concatenate(ps)
-
-
-def _merge_shared(parts, nout):
-    """Group pairs of several ops that write the same output blocks.
-
-    Each part's pairs are in task order; the stable sort keeps, inside every
-    output block, the ops in chunk order (fixed summation order)."""
-    rows, pairs = [], []
-    for t, p in parts:
-        cnt = (t['p_end'] - t['p_begin']).astype(np.int64)
-        rows.append(np.repeat(np.arange(len(t)), cnt))
-        pairs.append(p)
-    rows = np.concatenate(rows)
-    pairs = np.concatenate(pairs)
+def _merge_shared_many(items, nout):
+    """Instantiate the ops of ``items`` (every op's tasks are the same nout
+    output blocks) and group their pairs per output block; the stable sort
+    keeps, inside every output block, the ops in chunk order (fixed
+    summation order)."""
+    t, p = _instantiate_many(items)
+    nt = np.array([len(it[0]['tasks']) for it in items], dtype=np.int64)
+    cnt = (t['p_end'] - t['p_begin']).astype(np.int64)
+    local = np.arange(len(t), dtype=np.int64) - np.repeat(np.concatenate([[0], np.cumsum(nt)[:-1]]), nt)
+    rows = np.repeat(local, cnt)
     order = np.argsort(rows, kind='stable')
-    pairs = pairs[order]
     counts = np.bincount(rows, minlength=nout)
-    tasks = parts[0][0].copy()
+    tasks = t[:nt[0]].copy()
     tasks['p_begin'] = np.concatenate([[0], np.cumsum(counts)[:-1]])
     tasks['p_end'] = tasks['p_begin'] + counts
-    return tasks, pairs
+    return tasks, p[order]
 
 
 def _chunks(sizes, budget):
@@ -232,21 +235,21 @@ class SigmaProgram:
                 lo = lenv.off[p]
                 wo, vo = mpo.off[isite][p], mpo.off[jsite][p]
                 ro = renv.off[p]
-                shift = [sum(len(x[1]) for x in parts[k]) for k in range(4)]
-                parts[0].append(_instantiate(pl[0], lo, 0, offs[0], 1.0, shift[0]))
-                parts[1].append(_instantiate(pl[1], offs[0], wo, offs[1], 1.0, shift[1]))
-                parts[2].append(_instantiate(pl[2], offs[1], vo, offs[2], 1.0, shift[2]))
-                parts[3].append(_instantiate(pl[3], offs[2], ro, 0, float(hpwts[p]), 0))
+                parts[0].append((pl[0], lo, 0, offs[0], 1.0))
+                parts[1].append((pl[1], offs[0], wo, offs[1], 1.0))
+                parts[2].append((pl[2], offs[1], vo, offs[2], 1.0))
+                parts[3].append((pl[3], offs[2], ro, 0, float(hpwts[p])))
                 if self.fused:
                     inst.append((self.case_of[p], offs[0], poff_i[p], poff_j[p], offs[2]))
                 for k in range(3):
                     offs[k] += sizes[k]
             progs = []
             for k in range(3):
-                t, pr = _concat(parts[k])
+                t, pr = _instantiate_many(parts[k])
                 # L x psi: tall bra x MPO-bond rows, few columns
                 progs.append(Program(t, pr, prefer=SIGMA_STAGE1 if k == 0 else None))
-            t, pr = _merge_shared(parts[3], qtmp.nblks_allowed) if parts[3] else _concat([])
+            t, pr = (_merge_shared_many(parts[3], qtmp.nblks_allowed) if parts[3]
+                     else _instantiate_many([]))
             # sigma blocks with the same right sector read the same R blocks
             progs.append(Program(t, pr, reuse_key=qtmp.blk_isyms[:, 3] if len(t) else None,
                                  prefer=SIGMA_STAGE4))
@@ -369,12 +372,34 @@ def _diag_gather_index(meta):
     return idx, zd, bdim
 
 
+def _diag_mpo_factor(mpo, isite, p):
+    """P[n1 n2][w, wr2] = sum_wr Wi_p[w, wr, n1, n1] Wj_p[wr, wr2, n2, n2] over the
+    zero-QN rows of W[i] and columns of W[i+1] (None when it vanishes).  Depends
+    only on the MPO, so it is computed once per (site, op) and kept with it."""
+    cache = mpo.__dict__.setdefault('_diag_factor', {})
+    key = (isite, p)
+    if key not in cache:
+        cop, wop = mpo.sites[isite][p], mpo.sites[isite + 1][p]
+        zc = _zero_sector(cop.qsyms[0])
+        zw = _zero_sector(wop.qsyms[1])
+        P = None
+        if zc >= 0 and zw >= 0:
+            ci = cop.idlst[0][zc]
+            wi = wop.idlst[1][zw]
+            cdz = np.einsum('abnn->abn', cop.toDenseTensor()[ci])      # [w, wr, n]
+            wdz = np.einsum('abnn->abn', wop.toDenseTensor()[:, wi])   # [wr, wr2, n]
+            P = np.einsum('awn,wbm->nmab', cdz, wdz).reshape(16, len(ci), len(wi))
+            if not np.any(P):
+                P = None
+        cache[key] = P
+    return cache[key]
+
+
 class DiagProgram:
     """HDiagQt: diag[l,n1,n2,r] = sum_p hp sum_{w,wr,wr2} Ld_p[w,l] Wi_p[w,wr,n1,n1]
     Wj_p[wr,wr2,n2,n2] Rd_p[wr2,r], on the psi template's allowed blocks."""
 
     def __init__(self, mpo, isite, qtmp, lenv, renv, hpwts):
-        jsite = isite + 1
         nops = mpo.nops
         self.ndim0 = qtmp.size_allowed
         lgi, rgi, pmats = [], [], []
@@ -392,19 +417,8 @@ class DiagProgram:
             if li is None or ri is None:
                 continue
             assert ld == ldim and rd == rdim
-            cop, wop = mpo.sites[isite][p], mpo.sites[jsite][p]
-            cd = cop.toDenseTensor()
-            wd = wop.toDenseTensor()
-            zc = _zero_sector(cop.qsyms[0])
-            zw = _zero_sector(wop.qsyms[1])
-            if zc < 0 or zw < 0:
-                continue
-            ci = cop.idlst[0][zc]
-            wi = wop.idlst[1][zw]
-            cdz = np.einsum('abnn->abn', cd[ci])                 # [w, wr, n]
-            wdz = np.einsum('abnn->abn', wd[:, wi])              # [wr, wr2, n]
-            P = np.einsum('awn,wbm->nmab', cdz, wdz).reshape(16, len(ci), len(wi))
-            if not np.any(P):
+            P = _diag_mpo_factor(mpo, isite, p)
+            if P is None:
                 continue
             lgi.append(li.reshape(-1) + np.where(li.reshape(-1) >= 0, lenv.off[p], 0))
             rgi.append(ri.reshape(-1) + np.where(ri.reshape(-1) >= 0, renv.off[p], 0))
@@ -512,19 +526,13 @@ class RenormProgram:
             parts = [[], [], []]
             for p in ch:
                 pl = plans[case_of[p]]
-                shift = [sum(len(x[1]) for x in parts[k]) for k in range(3)]
-                if status == 'L':
-                    parts[0].append(_instantiate(pl[0], 0, env_in.off[p], offs[0], 1.0, shift[0]))
-                    parts[1].append(_instantiate(pl[1], mpo.off[site_idx][p], offs[0], offs[1],
-                                                 1.0, shift[1]))
-                else:
-                    parts[0].append(_instantiate(pl[0], 0, env_in.off[p], offs[0], 1.0, shift[0]))
-                    parts[1].append(_instantiate(pl[1], mpo.off[site_idx][p], offs[0], offs[1],
-                                                 1.0, shift[1]))
-                parts[2].append(_instantiate(pl[2], offs[1], 0, self.out_off[p], 1.0, shift[2]))
+                # the same operand roles in both directions (site, env | MPO, t1 | t2, site)
+                parts[0].append((pl[0], 0, env_in.off[p], offs[0], 1.0))
+                parts[1].append((pl[1], mpo.off[site_idx][p], offs[0], offs[1], 1.0))
+                parts[2].append((pl[2], offs[1], 0, self.out_off[p], 1.0))
                 offs[0] += pl[3].size_allowed
                 offs[1] += pl[4].size_allowed
-            progs = [Program(*_concat(parts[k])) for k in range(3)]
+            progs = [Program(*_instantiate_many(parts[k])) for k in range(3)]
             self.flops += sum(pg.flops for pg in progs)
             self.stages.append(progs)
             self.inter = [max(a, b) for a, b in zip(self.inter, offs)]

@@ -17,18 +17,38 @@ class DavidsonError(RuntimeError):
 
 
 class _VecOps:
+    MAXD = 64   # rows per dots call (workspace size)
+
     def __init__(self, n):
         self.n = n
-        self.work = dv.empty(_lib.lib().zmpo_dots_workspace(64), 0)
-        self.out = dv.empty(64, 0)
+        self.work = dv.empty(_lib.lib().zmpo_dots_workspace(self.MAXD), 0)
+        self.out = dv.empty(self.MAXD * self.MAXD, 0)
+
+    def _dots_into(self, X, nx, y, dst):
+        _lib.check(_lib.lib().zmpo_dots(0, dv.ptr(X), X.stride(0), nx, dv.ptr(y), self.n,
+                                        dv.ptr(dst), dv.ptr(self.work), dv.stream()))
 
     def dots(self, X, nx, y):
         """[X_j . y for j < nx] as a host array (X rows of a 2-D device tensor)."""
         if nx == 0:
             return np.zeros(0)
-        _lib.check(_lib.lib().zmpo_dots(0, dv.ptr(X), X.stride(0), nx, dv.ptr(y), self.n,
-                                        dv.ptr(self.out), dv.ptr(self.work), dv.stream()))
+        self._dots_into(X, nx, y, self.out)
         return dv.download(self.out[:nx]).copy()
+
+    def gram(self, X, nx, Y, ny):
+        """[[X_j . Y_i for j < nx] for i < ny] with one device-to-host copy."""
+        for i in range(ny):
+            self._dots_into(X, nx, Y[i], self.out[i * nx:])
+        return dv.download(self.out[:nx * ny]).reshape(ny, nx).copy()
+
+    def project_out(self, r, X, nx):
+        """r -= sum_j (X_j . r) X_j, the coefficients staying on the device."""
+        if nx == 0:
+            return r
+        self._dots_into(X, nx, r, self.out)
+        _lib.check(_lib.lib().zmpo_lincomb(0, dv.ptr(r), dv.ptr(X), X.stride(0), nx,
+                                           dv.ptr(self.out), -1.0, 1.0, self.n, dv.stream()))
+        return r
 
     def lincomb(self, out, X, coef, alpha=1.0, beta=0.0):
         nx = len(coef)
@@ -97,8 +117,7 @@ class EigenSolver:
         """Orthonormalise R[rows] against V[:nb] (5 passes) into V[out_start:]."""
         for _ in range(5):
             for i in rows:
-                c = ops.dots(V, nb, R[i])
-                ops.lincomb(R[i], V, c, -1.0, 1.0)
+                ops.project_out(R[i], V, nb)
         nindp = 0
         for pos, i in enumerate(rows):
             rii = ops.norm(R[i])
@@ -111,10 +130,8 @@ class EigenSolver:
             nindp += 1
             for _ in range(5):
                 for j in rows[pos:]:
-                    c = ops.dots(V, nb, R[j])
-                    ops.lincomb(R[j], V, c, -1.0, 1.0)
-                    c2 = ops.dots(V[out_start:], nindp, R[j])
-                    ops.lincomb(R[j], V[out_start:], c2, -1.0, 1.0)
+                    ops.project_out(R[j], V, nb)
+                    ops.project_out(R[j], V[out_start:], nindp)
         return nindp
 
     def solve_iter(self, v0=None):
@@ -151,10 +168,10 @@ class EigenSolver:
         vr = None
         for niter in range(1, self.maxcycle):
             self.niter = niter
-            S = np.array([ops.dots(V, nb, V[i]) for i in range(nb)])
+            S = ops.gram(V, nb, V, nb)
             if np.linalg.norm(S - np.identity(nb)) > 1e-10:
                 raise DavidsonError('basis not orthonormal: %g' % np.linalg.norm(S - np.identity(nb)))
-            H = np.array([ops.dots(W, nb, V[i]) for i in range(nb)])
+            H = ops.gram(W, nb, V, nb)
             if np.linalg.norm(H - H.T) > self.herm_tol * max(1.0, np.abs(H).max()):
                 raise DavidsonError('projected H not Hermitian: %g' % np.linalg.norm(H - H.T))
             H = 0.5 * (H + H.T)

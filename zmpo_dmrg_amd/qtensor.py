@@ -464,8 +464,10 @@ def make_schedule(tasks, pairs, rb, kchunk=2048, reuse_key=None, tile=None):
     tn = np.maximum((N + TNs - 1) // TNs, 1)
     cnt = pe - pb
     tp = np.repeat(np.arange(nt), cnt)
-    pidx = np.concatenate([np.arange(b, e) for b, e in zip(pb[cnt > 0], pe[cnt > 0])]) \
-        if np.any(cnt > 0) else np.zeros(0, np.int64)
+    # pair indices task by task: p_begin[t] + 0 .. cnt[t]-1 (vectorised ranges)
+    tot = int(cnt.sum())
+    pidx = np.repeat(pb, cnt) + (np.arange(tot, dtype=np.int64)
+                                 - np.repeat(np.cumsum(cnt) - cnt, cnt))
     if len(pidx):
         kk = K[pidx]
         csum = np.cumsum(kk)
