@@ -47,6 +47,7 @@ def run_instance(spec, nsweeps, dtype):
     d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=D, tol=1e-8, dtype=dtype)
     dv.sync()
     setup = time.time() - t0
+    print('[batch]   instance %d K=%d D=%d set up in %.1fs' % (i, K, D, setup), flush=True)
     energies, times = [], []
     for isw in range(nsweeps):
         t = time.time()
@@ -80,18 +81,33 @@ def summarize(results, wall, nsweeps):
             'p99_sweep_s': float(np.percentile(per, 99)) if len(per) else None}
 
 
-def run_batch(specs, workers, nsweeps, dtype):
+THREAD_VARS = ('OMP_NUM_THREADS', 'OPENBLAS_NUM_THREADS', 'MKL_NUM_THREADS')
+
+
+def run_batch(specs, workers, nsweeps, dtype, cores=16):
     """Run the instances on ``workers`` processes (spawned: each initialises
-    its own HIP context); returns (results sorted by index, summary)."""
+    its own HIP context) with cores // workers host threads each, so the
+    workers' BLAS and torch thread pools do not oversubscribe the ``cores``
+    host cores; returns (results sorted by index, summary)."""
     t = time.time()
     if workers <= 1:
         results = [_worker((s, nsweeps, dtype)) for s in specs]
     else:
-        ctx = mp.get_context('spawn')
-        with ctx.Pool(workers) as pool:
-            # largest instances first: shorter tail
-            order = sorted(specs, key=lambda s: -(s[1] * s[2] * s[2]))
-            results = list(pool.imap_unordered(_worker, [(s, nsweeps, dtype) for s in order]))
+        saved = {k: os.environ.get(k) for k in THREAD_VARS}
+        for k in THREAD_VARS:   # read by the spawned interpreters at start-up
+            os.environ[k] = str(max(1, cores // workers))
+        try:
+            ctx = mp.get_context('spawn')
+            with ctx.Pool(workers) as pool:
+                # largest instances first: shorter tail
+                order = sorted(specs, key=lambda s: -(s[1] * s[2] * s[2]))
+                results = list(pool.imap_unordered(_worker, [(s, nsweeps, dtype) for s in order]))
+        finally:
+            for k, v in saved.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
     wall = time.time() - t
     results.sort(key=lambda r: r['index'])
     return results, summarize(results, wall, nsweeps)
@@ -104,12 +120,13 @@ def main(argv=None):
     ap.add_argument('--sweeps', type=int, default=2)
     ap.add_argument('--dtype', type=int, default=1, help='0 = FP64, 1 = FP32 sigma + FP64 correction')
     ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--cores', type=int, default=16, help='host cores shared by the workers')
     args = ap.parse_args(argv)
     if args.workers > 16:
         print('at most 16 worker processes share one GPU', file=sys.stderr)
         return 2
     specs = instances(args.instances, args.seed)
-    results, summ = run_batch(specs, args.workers, args.sweeps, args.dtype)
+    results, summ = run_batch(specs, args.workers, args.sweeps, args.dtype, args.cores)
     summ.update({'workers': args.workers, 'dtype': args.dtype,
                  'energies': {r['index']: r['energies'][-1] for r in results}})
     print(json.dumps(summ))

@@ -1022,7 +1022,8 @@ def chain_structure(first, second, mid_key, out_key, mid_off, b1_map, b2_map):
     return groups, tasks, pairs, blocks
 
 
-CHAIN_CLASSES = (96, 160, 256)   # LDS size classes (A-tile + intermediate columns)
+# LDS size classes (A-tile + intermediate columns): one launch per class
+CHAIN_CLASSES = tuple(int(x) for x in os.environ.get('ZMPO_CHAIN_CLASSES', '96,160,256').split(','))
 CHAIN_ATILE = os.environ.get('ZMPO_CHAIN_ATILE', '0')   # '1', '0' or 'auto' (by class)
 
 
