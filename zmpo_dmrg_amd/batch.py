@@ -36,15 +36,18 @@ def instances(n=48, seed=0):
     return [(i, int(k), int(d), int(seed) * 1000 + i) for i, (k, d) in enumerate(zip(ks, ds))]
 
 
-def run_instance(spec, nsweeps, dtype):
-    """Sweeps of one instance; returns its energies and per-sweep wall times."""
+def run_instance(spec, nsweeps, dtype, budget_gb=6.0):
+    """Sweeps of one instance; returns its energies and per-sweep wall times.
+    ``budget_gb`` caps the sigma intermediates per op chunk, so that several
+    workers' instances fit in device memory together."""
     from . import device as dv
     from . import dmrg as pdmrg
     from .integrals import gen_integrals
     i, K, D, iseed = spec
     t0 = time.time()
     i1, i2 = gen_integrals(K, seed=iseed)
-    d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=D, tol=1e-8, dtype=dtype)
+    d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=D, tol=1e-8, dtype=dtype,
+                      budget=int(budget_gb * 2 ** 30))
     dv.sync()
     setup = time.time() - t0
     print('[batch]   instance %d K=%d D=%d set up in %.1fs' % (i, K, D, setup), flush=True)
@@ -58,13 +61,17 @@ def run_instance(spec, nsweeps, dtype):
         times.append(time.time() - t)
         energies.append(min(r1[2], r2[2]))
         print('[batch]   instance %d sweep %d L E=%.8f' % (i, isw, r2[2]), flush=True)
+    nsigma = d.nsigma
+    # hand the cached blocks back: workers share one device's memory
+    del d
+    dv.torch().cuda.empty_cache()
     return {'index': i, 'K': K, 'D': D, 'seed': iseed, 'energies': energies,
-            'sweep_s': times, 'setup_s': setup, 'nsigma': d.nsigma, 'pid': os.getpid()}
+            'sweep_s': times, 'setup_s': setup, 'nsigma': nsigma, 'pid': os.getpid()}
 
 
 def _worker(args):
-    spec, nsweeps, dtype = args
-    r = run_instance(spec, nsweeps, dtype)
+    spec, nsweeps, dtype, budget_gb = args
+    r = run_instance(spec, nsweeps, dtype, budget_gb)
     print('[batch] instance %d K=%d D=%d E=%.8f sweep %.2fs (pid %d)' % (
         r['index'], r['K'], r['D'], r['energies'][-1], float(np.mean(r['sweep_s'])), r['pid']),
         flush=True)
@@ -84,14 +91,14 @@ def summarize(results, wall, nsweeps):
 THREAD_VARS = ('OMP_NUM_THREADS', 'OPENBLAS_NUM_THREADS', 'MKL_NUM_THREADS')
 
 
-def run_batch(specs, workers, nsweeps, dtype, cores=16):
+def run_batch(specs, workers, nsweeps, dtype, cores=16, budget_gb=6.0):
     """Run the instances on ``workers`` processes (spawned: each initialises
     its own HIP context) with cores // workers host threads each, so the
     workers' BLAS and torch thread pools do not oversubscribe the ``cores``
     host cores; returns (results sorted by index, summary)."""
     t = time.time()
     if workers <= 1:
-        results = [_worker((s, nsweeps, dtype)) for s in specs]
+        results = [_worker((s, nsweeps, dtype, budget_gb)) for s in specs]
     else:
         saved = {k: os.environ.get(k) for k in THREAD_VARS}
         for k in THREAD_VARS:   # read by the spawned interpreters at start-up
@@ -101,7 +108,8 @@ def run_batch(specs, workers, nsweeps, dtype, cores=16):
             with ctx.Pool(workers) as pool:
                 # largest instances first: shorter tail
                 order = sorted(specs, key=lambda s: -(s[1] * s[2] * s[2]))
-                results = list(pool.imap_unordered(_worker, [(s, nsweeps, dtype) for s in order]))
+                results = list(pool.imap_unordered(
+                    _worker, [(s, nsweeps, dtype, budget_gb) for s in order]))
         finally:
             for k, v in saved.items():
                 if v is None:
@@ -121,12 +129,15 @@ def main(argv=None):
     ap.add_argument('--dtype', type=int, default=1, help='0 = FP64, 1 = FP32 sigma + FP64 correction')
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--cores', type=int, default=16, help='host cores shared by the workers')
+    ap.add_argument('--budget-gb', type=float, default=6.0,
+                    help='sigma intermediates per op chunk and instance (GiB)')
     args = ap.parse_args(argv)
     if args.workers > 16:
         print('at most 16 worker processes share one GPU', file=sys.stderr)
         return 2
     specs = instances(args.instances, args.seed)
-    results, summ = run_batch(specs, args.workers, args.sweeps, args.dtype, args.cores)
+    results, summ = run_batch(specs, args.workers, args.sweeps, args.dtype, args.cores,
+                              args.budget_gb)
     summ.update({'workers': args.workers, 'dtype': args.dtype,
                  'energies': {r['index']: r['energies'][-1] for r in results}})
     print(json.dumps(summ))

@@ -129,6 +129,11 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
             solver.herm_tol = 1e-5
         eig, vec, nmvp = solver.solve_iter(v0)
         if dtype:
+            # the low-precision copies of this step's environments are not
+            # needed again (the next step at this bond rebuilds them): drop
+            # them so that only the FP64 environments stay resident
+            for env in (fL, fR):
+                env._buf32 = None
             fine = EigenSolver()
             fine.crit_e, fine.crit_vec = dmrg.crit_e, dmrg.crit_vec
             fine.ndim, fine.neig, fine.diag = ndim0, neig, solver.diag

@@ -556,9 +556,9 @@ class RenormProgram:
         return EnvStore(self.meta, out, self.out_off)
 
 
-def renorm(mpo, site_idx, qsite, env_in, status, dtype=0):
+def renorm(mpo, site_idx, qsite, env_in, status, dtype=0, budget=DEFAULT_BUDGET):
     """One renormalisation step; qsite.value may be host or device."""
-    prog = RenormProgram(mpo, site_idx, qsite, env_in, status)
+    prog = RenormProgram(mpo, site_idx, qsite, env_in, status, budget)
     v = qsite.value
     v = dv.upload(v, dtype) if isinstance(v, np.ndarray) else (v if dtype == 0 else v.float())
     return prog.apply(mpo, v, env_in, dtype), prog
@@ -645,6 +645,7 @@ def renormQt_H00(dmrg, isite, ncsite, flst, flstN, site, status):
     # correction of a low-precision Davidson solve needs an H_eff that is
     # symmetric and accurate to FP64 (FP32-rounded environments leave a
     # ~1e-7 relative asymmetry that bounds the corrected energy)
-    new, _ = renorm(dmrg.fhop, idx, site, env_in, status, getattr(dmrg, 'renorm_dtype', 0))
+    new, _ = renorm(dmrg.fhop, idx, site, env_in, status, getattr(dmrg, 'renorm_dtype', 0),
+                    getattr(dmrg, 'budget', DEFAULT_BUDGET))
     flstN[0][0] = new
     return 0
