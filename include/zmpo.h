@@ -235,6 +235,17 @@ int zmpo_chaingemm(int dtype, int rb, int atile, int kcols_max, int ncols_max,
                    const void* A, const void* B1, const void* B2, void* C, int beta,
                    void* stream);
 
+/* zmpo_chaingemm with 16-column output tiles (one 16x16 MFMA block per
+ * tile instead of two): the same results bit for bit, for any N; half the
+ * MFMA work for row groups whose blocks all have N <= 16 (sigma at K <= 24,
+ * where MPO bond sectors are 11-13 wide).  Same contraction as
+ * zmpo_chaingemm (kernelQt.py:277-296). */
+int zmpo_chaingemm16(int dtype, int rb, int atile, int kcols_max, int ncols_max,
+                     const zmpo_cunit* units, int64_t nunits, const zmpo_cgroup* groups,
+                     const zmpo_cblock* blocks, const zmpo_task* tasks, const zmpo_pair* pairs,
+                     const void* A, const void* B1, const void* B2, void* C, int beta,
+                     void* stream);
+
 /* out[j] = sum_i X[j*ldx + i] * y[i] for j < nx (device out, float64;
  * work holds zmpo_dots_workspace(nx) doubles).  Fixed reduction order. */
 int zmpo_dots(int dtype, const void* X, int64_t ldx, int nx, const void* y, int64_t n,

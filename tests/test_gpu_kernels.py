@@ -177,12 +177,15 @@ def test_errors_surface(gpu):
     assert lib.zmpo_last_error()
 
 
+@pytest.mark.parametrize('n16', [0, 1])
 @pytest.mark.parametrize('atile', [1, 0])
 @pytest.mark.parametrize('dtype', [0, 1])
 @pytest.mark.parametrize('rb', [1, 2])
-def test_chaingemm_real_plan_matches_two_stages(gpu, rb, dtype, atile):
-    """zmpo_chaingemm (sigma stages 2-3 with the intermediate in LDS) vs the
-    CPU model of the two separate stages, on a real K=5 sigma plan."""
+def test_chaingemm_real_plan_matches_two_stages(gpu, rb, dtype, atile, n16):
+    """zmpo_chaingemm / zmpo_chaingemm16 (sigma stages 2-3 with the
+    intermediate in LDS; 32- or 16-column tiles, every launch forced to one
+    of them) vs the CPU model of the two separate stages, on a real K=5 sigma
+    plan."""
     from kmodel import host_model
     from zmpo_dmrg_amd import kernelQt as kq
     case5 = Case(K=5, D=24, seed=4, sweeps=1, nelec=4)
@@ -197,6 +200,7 @@ def test_chaingemm_real_plan_matches_two_stages(gpu, rb, dtype, atile):
         ch = ch.with_rb(rb)
         for l_ in ch.launches:   # A staged in LDS (1) or read from global memory (0)
             l_['atile'] = atile
+            l_['n16'] = bool(n16)
         n1 = int(st[1].pairs_host['a_off'].max() + 5000)
         n2 = int(st[1].tasks_host['c_off'].max() + 5000)
         n3 = int(st[2].tasks_host['c_off'].max() + 5000)
@@ -212,3 +216,33 @@ def test_chaingemm_real_plan_matches_two_stages(gpu, rb, dtype, atile):
         tol = 1e-12 if dtype == 0 else 2e-5
         err = np.abs(dv.download(C).astype(np.float64) - want).max()
         assert err < tol * max(1.0, np.abs(want).max()), (rb, dtype, err)
+
+
+def test_chaingemm16_bitwise_equal_to_chaingemm(gpu):
+    """The 16-column tiles change which MFMA block holds a column, not the
+    per-element summation order: FP64 results are bit-identical (K=6 plan)."""
+    from kmodel import host_model
+    from zmpo_dmrg_amd import kernelQt as kq
+    case6 = Case(K=6, D=32, seed=2, sweeps=1)
+    with host_model():
+        Lh, Rh = case6.product_envs()
+    isite = 2
+    qkey, _, pt, _ = case6.templates(isite)
+    sp = kq.SigmaProgram(case6.mpo, isite, pt, qkey, Lh[isite - 1], Rh[isite + 2],
+                         case6.mpo.hpwts, fused=True)
+    rng = np.random.default_rng(7)
+    for st, ch in zip(sp.stages, sp.chains):
+        n1 = int(st[1].pairs_host['a_off'].max() + 5000)
+        n3 = int(st[2].tasks_host['c_off'].max() + 5000)
+        A = dv.upload(rng.standard_normal(n1), 0)
+        Pi = dv.upload(case6.mpo.padded(isite)[0], 0)
+        Pj = dv.upload(case6.mpo.padded(isite + 1)[0], 0)
+        out = []
+        for flag in (False, True):
+            for l_ in ch.launches:
+                l_['n16'] = flag
+            C = dv.zeros(n3, 0)
+            pq.run_chain(ch, A, Pi, Pj, C, 0)
+            out.append(dv.download(C))
+        assert np.array_equal(out[0], out[1])
+        assert np.abs(out[0]).max() > 0

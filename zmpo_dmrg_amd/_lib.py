@@ -40,7 +40,7 @@ SUNIT_DTYPE = np.dtype([('c_off', '<i8'), ('c_smhi', '<i8'), ('c_mlo', '<i4'), (
 TILE = 64
 
 EXPORTS = ['zmpo_last_error', 'zmpo_abi_version', 'zmpo_struct_sizes', 'zmpo_symAllowed',
-           'zmpo_symAllowedInt', 'zmpo_blockgemm', 'zmpo_wavegemm', 'zmpo_shapedgemm', 'zmpo_streamgemm', 'zmpo_chaingemm', 'zmpo_skinnygemm', 'zmpo_dotgemm', 'zmpo_dots', 'zmpo_dots_workspace',
+           'zmpo_symAllowedInt', 'zmpo_blockgemm', 'zmpo_wavegemm', 'zmpo_shapedgemm', 'zmpo_streamgemm', 'zmpo_chaingemm', 'zmpo_chaingemm16', 'zmpo_skinnygemm', 'zmpo_dotgemm', 'zmpo_dots', 'zmpo_dots_workspace',
            'zmpo_lincomb', 'zmpo_precond', 'zmpo_gather', 'zmpo_scatter', 'zmpo_convert']
 
 
@@ -77,6 +77,7 @@ def lib():
     L.zmpo_skinnygemm.argtypes = [i32, i32, vp, vp, i64, vp, vp, vp, vp, i32, vp]
     L.zmpo_chaingemm.argtypes = [i32, i32, i32, i32, i32, vp, i64, vp, vp, vp, vp, vp, vp, vp,
                                  vp, i32, vp]
+    L.zmpo_chaingemm16.argtypes = L.zmpo_chaingemm.argtypes
     L.zmpo_streamgemm.argtypes = [i32, i32, i32, vp, i64, i32, vp, vp, i64, vp, vp, vp, vp, vp,
                                   i32, vp]
     L.zmpo_dots.argtypes = [i32, vp, i64, i32, vp, i64, vp, vp, vp]

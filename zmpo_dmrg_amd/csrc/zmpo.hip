@@ -1147,7 +1147,7 @@ struct ChainRows {
 // n0.  Only k-steps that start below K are loaded (uniform test); lanes past
 // K inside the last k-step read the zero padding of both operands.  Bases
 // are wave-uniform (scalar registers); each lane adds a fixed 32-bit offset.
-template <typename T, int RB, bool GA>
+template <typename T, int RB, bool GA, bool N16>
 __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPair& pr, int n0,
                                            int k0, const T* __restrict__ src,
                                            const T* __restrict__ Bb, int lane_a, int lr, int lk,
@@ -1170,7 +1170,7 @@ __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPai
       if (k0 + 4 * s < pr.K) {
         const T* __restrict__ bs = b0 + 4 * s * pr.ldb;
         sl.b[s][0] = bs[lane_b];
-        sl.b[s][1] = bs[lane_b + 16];
+        if (!N16) sl.b[s][1] = bs[lane_b + 16];
         const int64_t ko = (int64_t)min(k0 + 4 * s + lk, pr.K - 1) * bk.a_sk;
 #pragma unroll
         for (int i = 0; i < RB; ++i) sl.a[s][i] = ab[ro[i] + ko];
@@ -1184,7 +1184,7 @@ __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPai
     if (k0 + 4 * s < pr.K) {
       const T* __restrict__ bs = b0 + 4 * s * pr.ldb;
       sl.b[s][0] = bs[lane_b];
-      sl.b[s][1] = bs[lane_b + 16];
+      if (!N16) sl.b[s][1] = bs[lane_b + 16];
       const T* __restrict__ as = a0 + 4 * s * LD;
 #pragma unroll
       for (int i = 0; i < RB; ++i) sl.a[s][i] = as[lane_a + i * 16];
@@ -1201,11 +1201,13 @@ __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPai
 // the operands of item i+1 (possibly of the next pair or task) while the
 // MFMAs of item i run; the two operand slices alternate roles, so nothing
 // is copied between them.
-template <typename T, int RB, bool TO_LDS, bool GA>
+template <typename T, int RB, bool TO_LDS, bool GA, bool N16>
 struct ChainWalk {
   typedef typename Acc<T>::type accv;
   static constexpr int LD = 16 * RB + 1;
   static constexpr int KD = 4 * CHAIN_KS;
+  static constexpr int TW = N16 ? 16 : 32;  // column tile width
+  static constexpr int NJ = N16 ? 1 : 2;    // 16-column MFMA blocks per tile
   const zmpo_task* __restrict__ tasks;
   const zmpo_pair* __restrict__ pairs;
   const T* __restrict__ src;
@@ -1225,7 +1227,7 @@ struct ChainWalk {
   // load the item under the cursor into sl; returns its k-step count
   __device__ __forceinline__ int load(ChainSlice<T, RB>& sl) {
     if (p >= tpe) return 0;
-    chain_load<T, RB, GA>(sl, pr, n0, k0, src, Bb, lane_a, lr, lk, Ag, blocks, rows);
+    chain_load<T, RB, GA, N16>(sl, pr, n0, k0, src, Bb, lane_a, lr, lk, Ag, blocks, rows);
     return min(CHAIN_KS, (pr.K - k0 + 3) / 4);
   }
   // move the cursor to the next item; returns whether the previous item
@@ -1245,7 +1247,7 @@ struct ChainWalk {
       return false;
     }
     p = tpb;
-    n0 += 32;
+    n0 += TW;
     if (n0 < tN) {
       if (p < tpe) pr.get(pairs, p);
       return true;
@@ -1268,7 +1270,7 @@ struct ChainWalk {
 #pragma unroll
       for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = Acc<T>::mfma(sl.a[s][i], sl.b[s][j], acc[i][j]);
+        for (int j = 0; j < NJ; ++j) acc[i][j] = Acc<T>::mfma(sl.a[s][i], sl.b[s][j], acc[i][j]);
     }
   }
   __device__ __forceinline__ void flush(int ct, int cn0, int cN, const int (&mb)[RB][4],
@@ -1283,7 +1285,7 @@ struct ChainWalk {
           if (rok[i][r]) {
             const int64_t ro = (int64_t)mb[i][r] * tk.c_smhi + (int64_t)mr[i][r] * tk.c_smlo;
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
+            for (int j = 0; j < NJ; ++j) {
               const int n = cn0 + j * 16 + lr;
               if (n < cN) {
                 T* q = cb + ro + (int64_t)n * tk.c_sn;
@@ -1297,7 +1299,7 @@ struct ChainWalk {
       const int co = tk.c_off;
       const int w4 = (cN + 3) & ~3;
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      for (int j = 0; j < NJ; ++j) {
         const int n = cn0 + j * 16 + lr;
         if (n >= w4) continue;
 #pragma unroll
@@ -1312,7 +1314,7 @@ struct ChainWalk {
   }
 };
 
-template <typename T, int RB, bool TO_LDS, bool GA>
+template <typename T, int RB, bool TO_LDS, bool GA, bool N16>
 __device__ __forceinline__ void chain_stage(
     int t0, int t1, const zmpo_task* __restrict__ tasks, const zmpo_pair* __restrict__ pairs,
     const T* __restrict__ src, const T* __restrict__ Bb, T* __restrict__ dst,
@@ -1320,7 +1322,7 @@ __device__ __forceinline__ void chain_stage(
     const bool (&rok)[RB][4], int beta, int wv, int lr, int lk, const T* __restrict__ Ag,
     const zmpo_cblock* __restrict__ blocks, const ChainRows<RB>& rows) {
   typedef typename Acc<T>::type accv;
-  ChainWalk<T, RB, TO_LDS, GA> w;
+  ChainWalk<T, RB, TO_LDS, GA, N16> w;
   w.Ag = Ag;
   w.blocks = blocks;
   w.rows = rows;
@@ -1367,7 +1369,7 @@ __device__ __forceinline__ void chain_stage(
   }
 }
 
-template <typename T, int RB, bool ATILE>
+template <typename T, int RB, bool ATILE, bool N16>
 __global__ __launch_bounds__(256) void chain_gemm_kernel(
     const zmpo_cunit* __restrict__ units, int64_t nunits, const zmpo_cgroup* __restrict__ groups,
     const zmpo_cblock* __restrict__ blocks, const zmpo_task* __restrict__ tasks,
@@ -1435,17 +1437,17 @@ __global__ __launch_bounds__(256) void chain_gemm_kernel(
   }
   if (ATILE) __syncthreads();
   // ---- stage 1: intermediate columns into LDS
-  chain_stage<T, RB, true, !ATILE>(g.t1_begin, g.t1_end, tasks, pairs, atile, B1 + un.b1_base,
+  chain_stage<T, RB, true, !ATILE, N16>(g.t1_begin, g.t1_end, tasks, pairs, atile, B1 + un.b1_base,
                                    mid, nullptr, 0, mb, mr, rok, 0, wv, lr, lk, A + un.a_base,
                                    blocks, arows);
   __syncthreads();
   // ---- stage 2: output blocks
-  chain_stage<T, RB, false, false>(g.t2_begin, g.t2_end, tasks, pairs, mid, B2 + un.b2_base,
+  chain_stage<T, RB, false, false, N16>(g.t2_begin, g.t2_end, tasks, pairs, mid, B2 + un.b2_base,
                                    nullptr, C, un.c_base, mb, mr, rok, beta, wv, lr, lk,
                                    nullptr, blocks, arows);
 }
 
-template <typename T, int RB, bool ATILE>
+template <typename T, int RB, bool ATILE, bool N16>
 int launch_chain_t(int kcols_max, int ncols_max, const zmpo_cunit* units, int64_t nunits,
                    const zmpo_cgroup* groups, const zmpo_cblock* blocks, const zmpo_task* tasks,
                    const zmpo_pair* pairs, const void* A, const void* B1, const void* B2, void* C,
@@ -1453,27 +1455,27 @@ int launch_chain_t(int kcols_max, int ncols_max, const zmpo_cunit* units, int64_
   const size_t lds = (size_t)((ATILE ? kcols_max : 0) + ncols_max) * (16 * RB + 1) * sizeof(T);
   if (lds > 160 * 1024) return set_error(ZMPO_ERR_ARG, "chaingemm: row tile exceeds 160 KiB of LDS");
   if (lds > 64 * 1024) {
-    int rc = check_hip(hipFuncSetAttribute((const void*)chain_gemm_kernel<T, RB, ATILE>,
+    int rc = check_hip(hipFuncSetAttribute((const void*)chain_gemm_kernel<T, RB, ATILE, N16>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                        "chaingemm: LDS attribute");
     if (rc) return rc;
   }
-  hipLaunchKernelGGL((chain_gemm_kernel<T, RB, ATILE>), dim3((unsigned)nunits), dim3(256), lds, s,
+  hipLaunchKernelGGL((chain_gemm_kernel<T, RB, ATILE, N16>), dim3((unsigned)nunits), dim3(256), lds, s,
                      units, nunits, groups, blocks, tasks, pairs, (const T*)A, (const T*)B1,
                      (const T*)B2, (T*)C, beta, kcols_max);
   return check_hip(hipGetLastError(), "chain_gemm launch");
 }
 
-template <typename T, int RB>
+template <typename T, int RB, bool N16>
 int launch_chain(int atile, int kcols_max, int ncols_max, const zmpo_cunit* units,
                  int64_t nunits, const zmpo_cgroup* groups, const zmpo_cblock* blocks,
                  const zmpo_task* tasks, const zmpo_pair* pairs, const void* A, const void* B1,
                  const void* B2, void* C, int beta, hipStream_t s) {
   if (atile)
-    return launch_chain_t<T, RB, true>(kcols_max, ncols_max, units, nunits, groups, blocks, tasks,
-                                       pairs, A, B1, B2, C, beta, s);
-  return launch_chain_t<T, RB, false>(kcols_max, ncols_max, units, nunits, groups, blocks, tasks,
-                                      pairs, A, B1, B2, C, beta, s);
+    return launch_chain_t<T, RB, true, N16>(kcols_max, ncols_max, units, nunits, groups, blocks,
+                                            tasks, pairs, A, B1, B2, C, beta, s);
+  return launch_chain_t<T, RB, false, N16>(kcols_max, ncols_max, units, nunits, groups, blocks,
+                                           tasks, pairs, A, B1, B2, C, beta, s);
 }
 
 // ---------------------------------------------------------------- vectors
@@ -1750,11 +1752,11 @@ int zmpo_skinnygemm(int dtype, int rb, const zmpo_task* tasks, const zmpo_unit* 
   return set_error(ZMPO_ERR_ARG, "skinnygemm: unsupported dtype or row-block count");
 }
 
-int zmpo_chaingemm(int dtype, int rb, int atile, int kcols_max, int ncols_max,
-                   const zmpo_cunit* units, int64_t nunits, const zmpo_cgroup* groups,
-                   const zmpo_cblock* blocks, const zmpo_task* tasks, const zmpo_pair* pairs,
-                   const void* A, const void* B1, const void* B2, void* C, int beta,
-                   void* stream) {
+static int chaingemm(bool n16, int dtype, int rb, int atile, int kcols_max, int ncols_max,
+                     const zmpo_cunit* units, int64_t nunits, const zmpo_cgroup* groups,
+                     const zmpo_cblock* blocks, const zmpo_task* tasks, const zmpo_pair* pairs,
+                     const void* A, const void* B1, const void* B2, void* C, int beta,
+                     void* stream) {
   if (nunits <= 0) return ZMPO_OK;
   if (!units || !groups || !blocks || !tasks || !pairs || !A || !B1 || !B2 || !C)
     return set_error(ZMPO_ERR_ARG, "chaingemm: null pointer");
@@ -1762,14 +1764,38 @@ int zmpo_chaingemm(int dtype, int rb, int atile, int kcols_max, int ncols_max,
   if (ncols_max < 1 || kcols_max < 1)
     return set_error(ZMPO_ERR_ARG, "chaingemm: kcols_max and ncols_max must be >= 1");
   hipStream_t s = (hipStream_t)stream;
+#define ZMPO_CHAIN(T, RB)                                                                      \
+  return n16 ? launch_chain<T, RB, true>(atile, kcols_max, ncols_max, units, nunits, groups,  \
+                                         blocks, tasks, pairs, A, B1, B2, C, beta, s)         \
+             : launch_chain<T, RB, false>(atile, kcols_max, ncols_max, units, nunits, groups, \
+                                          blocks, tasks, pairs, A, B1, B2, C, beta, s)
   if (dtype == ZMPO_F64) {
-    if (rb == 1) return launch_chain<double, 1>(atile, kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
-    if (rb == 2) return launch_chain<double, 2>(atile, kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
+    if (rb == 1) ZMPO_CHAIN(double, 1);
+    if (rb == 2) ZMPO_CHAIN(double, 2);
   } else if (dtype == ZMPO_F32) {
-    if (rb == 1) return launch_chain<float, 1>(atile, kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
-    if (rb == 2) return launch_chain<float, 2>(atile, kcols_max, ncols_max, units, nunits, groups, blocks, tasks, pairs, A, B1, B2, C, beta, s);
+    if (rb == 1) ZMPO_CHAIN(float, 1);
+    if (rb == 2) ZMPO_CHAIN(float, 2);
   }
+#undef ZMPO_CHAIN
   return set_error(ZMPO_ERR_ARG, "chaingemm: unsupported dtype or row-block count");
+}
+
+int zmpo_chaingemm(int dtype, int rb, int atile, int kcols_max, int ncols_max,
+                   const zmpo_cunit* units, int64_t nunits, const zmpo_cgroup* groups,
+                   const zmpo_cblock* blocks, const zmpo_task* tasks, const zmpo_pair* pairs,
+                   const void* A, const void* B1, const void* B2, void* C, int beta,
+                   void* stream) {
+  return chaingemm(false, dtype, rb, atile, kcols_max, ncols_max, units, nunits, groups, blocks,
+                   tasks, pairs, A, B1, B2, C, beta, stream);
+}
+
+int zmpo_chaingemm16(int dtype, int rb, int atile, int kcols_max, int ncols_max,
+                     const zmpo_cunit* units, int64_t nunits, const zmpo_cgroup* groups,
+                     const zmpo_cblock* blocks, const zmpo_task* tasks, const zmpo_pair* pairs,
+                     const void* A, const void* B1, const void* B2, void* C, int beta,
+                     void* stream) {
+  return chaingemm(true, dtype, rb, atile, kcols_max, ncols_max, units, nunits, groups, blocks,
+                   tasks, pairs, A, B1, B2, C, beta, stream);
 }
 
 int zmpo_streamgemm(int dtype, int shape, int kt, const zmpo_sunit* units, int64_t nunits,

@@ -1025,6 +1025,9 @@ def chain_structure(first, second, mid_key, out_key, mid_off, b1_map, b2_map):
 # LDS size classes (A-tile + intermediate columns): one launch per class
 CHAIN_CLASSES = tuple(int(x) for x in os.environ.get('ZMPO_CHAIN_CLASSES', '96,160,256').split(','))
 CHAIN_ATILE = os.environ.get('ZMPO_CHAIN_ATILE', '0')   # '1', '0' or 'auto' (by class)
+# 16-column output tiles (zmpo_chaingemm16) for row groups whose blocks are
+# all <= 16 columns wide: 'auto', '1' (every group) or '0' (none)
+CHAIN_N16 = os.environ.get('ZMPO_CHAIN_N16', 'auto')
 
 
 class ChainProgram:
@@ -1099,20 +1102,34 @@ class ChainProgram:
         width = (g['kcols'] + g['ncols']).astype(np.int64)
         cls = np.searchsorted(np.asarray(CHAIN_CLASSES), width[self.units_host['group']]) \
             if self.nunits else np.zeros(0, np.int64)
+        # narrow row groups: every block of both stages at most 16 columns wide
+        tN = self.tasks_host['N'].astype(np.int64)
+        wide = np.zeros(len(g), bool)
+        for lo, hi in (('t1_begin', 't1_end'), ('t2_begin', 't2_end')):
+            cnt = (g[hi] - g[lo]).astype(np.int64)
+            if len(tN) and cnt.sum():
+                gid = np.repeat(np.arange(len(g)), cnt)
+                tid = np.repeat(g[lo].astype(np.int64), cnt) + (
+                    np.arange(int(cnt.sum())) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+                wide |= np.bincount(gid, weights=(tN[tid] > 16), minlength=len(g)) > 0
+        n16 = {'1': np.ones(len(g), bool), '0': np.zeros(len(g), bool)}.get(CHAIN_N16, ~wide)
+        unit_n16 = n16[self.units_host['group']] if self.nunits else np.zeros(0, bool)
         self.launches = []
         for k in range(len(CHAIN_CLASSES) + 1):
-            sel = np.flatnonzero(cls == k)
-            if len(sel) == 0:
-                continue
-            us = self.units_host[sel]
-            gsel = us['group']
-            # A blocks read from global memory (default: the LDS then holds
-            # only the intermediate, so more workgroups fit per CU; measured
-            # faster at K=36, D=1000) or staged in LDS; 'auto' stages them
-            # for every class but the widest
-            atile = {'1': 1, '0': 0}.get(CHAIN_ATILE, int(k < len(CHAIN_CLASSES)))
-            self.launches.append({'units': us, 'kmax': int(max(1, g['kcols'][gsel].max())),
-                                  'nmax': int(max(1, g['ncols'][gsel].max())), 'atile': atile})
+            for nar in (False, True):
+                sel = np.flatnonzero((cls == k) & (unit_n16 == nar))
+                if len(sel) == 0:
+                    continue
+                us = self.units_host[sel]
+                gsel = us['group']
+                # A blocks read from global memory (default: the LDS then holds
+                # only the intermediate, so more workgroups fit per CU; measured
+                # faster at K=36, D=1000) or staged in LDS; 'auto' stages them
+                # for every class but the widest
+                atile = {'1': 1, '0': 0}.get(CHAIN_ATILE, int(k < len(CHAIN_CLASSES)))
+                self.launches.append({'units': us, 'kmax': int(max(1, g['kcols'][gsel].max())),
+                                      'nmax': int(max(1, g['ncols'][gsel].max())),
+                                      'atile': atile, 'n16': bool(nar)})
         self._dev = None
 
     def with_rb(self, rb):
@@ -1166,9 +1183,10 @@ def run_chain(prog, A, B1, B2, C, dtype=_lib.F64, beta=0, stream=None):
     st = dv.stream() if stream is None else stream
     L = _lib.lib()
     for l_, u in zip(prog.launches, units):
-        _lib.check(L.zmpo_chaingemm(dtype, prog.rb, l_['atile'], l_['kmax'], l_['nmax'], dv.ptr(u),
-                                    len(l_['units']), dv.ptr(g), dv.ptr(b), dv.ptr(t), dv.ptr(p),
-                                    dv.ptr(A), dv.ptr(B1), dv.ptr(B2), dv.ptr(C), beta, st))
+        fn = L.zmpo_chaingemm16 if l_.get('n16') else L.zmpo_chaingemm
+        _lib.check(fn(dtype, prog.rb, l_['atile'], l_['kmax'], l_['nmax'], dv.ptr(u),
+                      len(l_['units']), dv.ptr(g), dv.ptr(b), dv.ptr(t), dv.ptr(p),
+                      dv.ptr(A), dv.ptr(B1), dv.ptr(B2), dv.ptr(C), beta, st))
 
 
 def gather(dst, src, idx, dtype_dst=_lib.F64, dtype_src=_lib.F64):
