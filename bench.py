@@ -195,6 +195,8 @@ def main():
     ap.add_argument('--budget-gb', type=float, default=96.0)
     ap.add_argument('--cpu-ops', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-f32', action='store_true',
+                    help='skip the FP32-operand sigma timed after the FP64 one')
     ap.add_argument('--unfused', action='store_true',
                     help='run sigma stages 2 and 3 as two launches through HBM')
     ap.add_argument('--kernel', choices=['stream', 'shaped', 'wave', 'tile'], default=None,
@@ -262,6 +264,25 @@ def main():
             'stage_ms': [round(x, 4) for x in stage_ms],
             'stage_gflop': [round(x / 1e9, 3) for x in stage_fl],
             'stage_alg_gb': [round(x / 1e9, 3) for x in stage_by]}
+    # the same sigma in FP32 operands (the precision form's low-precision
+    # phase), after the timed region: its time and its error against FP64
+    prec = None
+    if dtype == 0 and not args.no_f32:
+        psi32 = dv.upload(psi_h, 1)
+        sig32 = dv.empty(st.qtmp.size_allowed, 1)
+        prog.apply(mpo, L, R, psi32, sig32, 1)
+        dv.sync()
+        t = time.time()
+        for _ in range(args.steps):
+            prog.apply(mpo, L, R, psi32, sig32, 1)
+        dv.sync()
+        ms32 = 1e3 * (time.time() - t) / args.steps
+        prog.apply(mpo, L, R, psi, sig, 0)
+        s64 = dv.download(sig)
+        err = float(np.linalg.norm(dv.download(sig32).astype(np.float64) - s64) / np.linalg.norm(s64))
+        prec = {'f32_ms_per_step': ms32, 'f32_gflops': prog.flops / ms32 / 1e6,
+                'f32_speedup_vs_f64': ms_step / ms32, 'f32_rel_err': err}
+        log('FP32 sigma: %.3f ms/step (%.2fx FP64), relative error %.2e' % (ms32, ms_step / ms32, err))
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -285,7 +306,7 @@ def main():
             args.K, args.K, args.D, mpo.nops), 'ndim0': int(st.qtmp.size_allowed),
             'sigma_gflop': prog.flops / 1e9, 'chunks': len(prog.chunks),
             'launches': ' | '.join(names)},
-        'roofline': roof, 'cpu_baseline': cpu,
+        'roofline': roof, 'cpu_baseline': cpu, 'precision': prec,
     }
     print(json.dumps(out), flush=True)
 
