@@ -144,7 +144,6 @@ def cpu_baseline(mpo, isite, st, L, R, nops_sample, psi_host):
     """Oracle HVecQt (numpy, the reference's per-block loop) on a subset of ops."""
     from oracle import kernels as ok
     from oracle.qtensor import QTensor as OQ
-    import oracle.qtensor as oq
 
     def conv(q):
         o = OQ(q.status)

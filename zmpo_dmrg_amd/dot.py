@@ -21,7 +21,6 @@ import numpy as np
 
 from . import device as dv
 from . import kernelQt
-from .mpo import qphys
 from .pdvdson import EigenSolver
 from .qtensor import QTensor, fromQnums
 
