@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, 'libzmpo.so')
+# ZMPO_LIB: load another build of the same sources (e.g. with other
+# compile-time tile constants) instead of the in-tree library
+LIB_PATH = os.environ.get('ZMPO_LIB') or os.path.join(_HERE, 'libzmpo.so')
 
 F64 = 0
 F32 = 1

@@ -1097,9 +1097,11 @@ class ChainProgram:
                                   self.groups_host['t2_end'][gl + len(g) - 1]].sum())
         self.units_host = np.concatenate(uparts) if uparts else np.zeros(0, _lib.CUNIT_DTYPE)
         self.nunits = len(self.units_host)
-        # LDS classes
+        # LDS classes, by the columns a group keeps in LDS: the intermediate,
+        # plus the A tile when it is staged there (CHAIN_ATILE '1')
         g = self.groups_host
-        width = (g['kcols'] + g['ncols']).astype(np.int64)
+        width = g['ncols'].astype(np.int64) + (g['kcols'].astype(np.int64)
+                                               if CHAIN_ATILE == '1' else 0)
         cls = np.searchsorted(np.asarray(CHAIN_CLASSES), width[self.units_host['group']]) \
             if self.nunits else np.zeros(0, np.int64)
         # narrow row groups: every block of both stages at most 16 columns wide
