@@ -17,18 +17,24 @@
  *                        sigma stages of HVecQt (source/mpo_dmrg_kernelQt.py:
  *                        270-306), the three renormalisation stages of
  *                        renormQt_H00 (kernelQt.py:674-683, 752-758) and
- *                        genHopsQt (source/mpo_dmrg_initQt.py:146-152), and
- *                        the final contraction of HDiagQt (kernelQt.py:116-119).
+ *                        genHopsQt (source/mpo_dmrg_initQt.py:146-152), the
+ *                        final contraction of HDiagQt (kernelQt.py:116-119),
+ *                        and the three stages of the energy gradient dE/dW
+ *                        behind the RDMs (properties/mpo_dmrg_propsItrf.py:
+ *                        204-263, computed as a derivative here).
  *   zmpo_wavegemm        the same numeric loop, scheduled one wave per small
  *                        output tile (the block shapes of block-sparse
  *                        DMRG are mostly 5-150 wide).
- *   zmpo_shapedgemm, zmpo_streamgemm, zmpo_skinnygemm
+ *   zmpo_shapedgemm, zmpo_streamgemm, zmpo_skinnygemm, zmpo_dotgemm
  *                        the same numeric loop with LDS-staged block-shaped
  *                        tiles (one tile per workgroup / unit ranges per
- *                        workgroup / tall narrow blocks).
- *   zmpo_chaingemm       the two middle stages of HVecQt (kernelQt.py:
+ *                        workgroup / tall narrow blocks / K-contiguous
+ *                        operands with the k range split over the waves).
+ *   zmpo_chaingemm, zmpo_chaingemm16
+ *                        the two middle stages of HVecQt (kernelQt.py:
  *                        277-296, top1 x W[i] then x W[i+1]) as one launch
- *                        with the intermediate top2 kept on chip.
+ *                        with the intermediate top2 kept on chip (32- or
+ *                        16-column output tiles).
  *   zmpo_dots, zmpo_lincomb, zmpo_precond
  *                        the vector algebra of the Davidson solver
  *                        (source/tools/pdvdson.py:131-283, 350-390), which the
