@@ -70,6 +70,7 @@ class MPODMRG:
         self.crit_e = tol
         self.crit_vec = crit_vec
         self.thresh = -1e-14
+        self.noise = 0.0     # perturbative noise of the decimation RDM (pRDMQt)
         self.dtype = dtype
         self.budget = budget
         self.psi0 = None

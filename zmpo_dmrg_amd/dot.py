@@ -147,7 +147,7 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
         nmvps += nmvp
         dmrg.niters.append(solver.niter)
     civecs = np.vstack(civecs)
-    sigs, dwts, qred, site, srotR = decimation(dmrg, civecs, isite, status)
+    sigs, dwts, qred, site, srotR = decimation(dmrg, civecs, isite, status, info)
     genGuess(dmrg, isite, actlst, civecs, srotR, status)
     return nmvps, np.array(eigs), sigs, dwts, qred, site, srotR
 
@@ -183,11 +183,13 @@ def canonical_null(u, s2, tol):
     return u2
 
 
-def rdm_blkdiag(mats, classes, thresh, Dcut, side):
+def rdm_blkdiag(mats, classes, thresh, Dcut, side, extra=None):
     """Per-sector RDM eigh and truncation (mpo_dmrg_qparser.py:101-180).
 
     ``mats`` are the C matrices (rows = kept side when side == 'L', columns
     when side == 'R'); the RDM of each sector is formed from its own rows only.
+    ``extra`` (dense, kept side x kept side) is added to every sector's RDM:
+    the noise term (dotutil.py:312-333), block diagonal in the sectors.
     """
     keys, groups = [], {}
     for idx, val in enumerate(classes):
@@ -207,6 +209,8 @@ def rdm_blkdiag(mats, classes, thresh, Dcut, side):
         else:
             sub = [m[:, rows] for m in mats]
             rho = sum(s.T @ s for s in sub)
+        if extra is not None:
+            rho = rho + extra[np.ix_(rows, rows)]
         tr += np.trace(rho)
         blocks.append((k, rows, rho))
     if tr < -1e-12:
@@ -246,20 +250,26 @@ def rdm_blkdiag(mats, classes, thresh, Dcut, side):
     return 1.0 - np.sum(sig_kept), np.array([qsym[i] for i in indx]), rot, sig_kept
 
 
-def decimation(dmrg, civecs, isite, status):
-    """dotutil.py:288-495 for ncsite=2, noise=0."""
+def decimation(dmrg, civecs, isite, status, info=None):
+    """dotutil.py:288-495 for ncsite=2.  With dmrg.noise >= 1e-10 (and the
+    solver's info) the RDM gets noise times the operator term of pRDMQt."""
     ldim, cdim, rdim, _ = dmrg.dims
     neig = civecs.shape[0]
     mats = civecs.reshape(neig, ldim * 4, 4 * rdim)
+    extra = None
+    noise = float(getattr(dmrg, 'noise', 0.0))
+    if noise >= 1e-10 and info is not None:
+        shp = (neig, ldim, 4, 4 * rdim) if status == 'L' else (neig, ldim * 4, 4, rdim)
+        extra = noise * kernelQt.prdm_noise(civecs.reshape(shp), info)
     if status == 'L':
         classes = dpt(dmrg.qnuml[isite], dmrg.qphys[isite])
-        dwts, qred, rot, sigs = rdm_blkdiag(mats, classes, dmrg.thresh, dmrg.Dmax, 'L')
+        dwts, qred, rot, sigs = rdm_blkdiag(mats, classes, dmrg.thresh, dmrg.Dmax, 'L', extra)
         nres = len(qred)
         site = rot.reshape(ldim, 4, nres).copy()
         srotR = np.array([_sparse_matmul(rot.T, m).reshape(nres, 4, rdim) for m in mats])
     else:
         classes = dpt(dmrg.qphys[isite + 1], dmrg.qnumr[isite + 2])
-        dwts, qred, rot, sigs = rdm_blkdiag(mats, classes, dmrg.thresh, dmrg.Dmax, 'R')
+        dwts, qred, rot, sigs = rdm_blkdiag(mats, classes, dmrg.thresh, dmrg.Dmax, 'R', extra)
         nres = len(qred)
         site = rot.reshape(4, rdim, nres).transpose(2, 0, 1).copy()
         srotR = np.array([_sparse_matmul(m, rot).reshape(ldim, 4, nres) for m in mats])

@@ -584,6 +584,172 @@ def genHops(mpo, sites, status, nqnum=2):
     return envs, exphop
 
 
+# ------------------------------------------------------------------ noise RDM
+class PRDMProgram:
+    """Noise term of the decimation RDM (pRDMQt, kernelQt.py:453-626):
+    sum_p X_p X_p^T with, for status 'L', X_p[l, u, m] = sum L_p[w, l, l']
+    W_p[i][w, wr, u, d] C[l', d, m] (m = (n2, r) with wr kept), and for 'R'
+    X_p[u, r, m] = sum W_p[j][wl, wr, u, d] R_p[wr, r, r'] C[m, d, r']
+    (m = (l, n1) with wl kept).  Three launches per op chunk; the Gram stage
+    of the ops of one case is one launch whose pairs are grouped per output
+    block (ops in a fixed order).  C is the 3-leg template of
+    dotutil.setupQtmpPRDM (dotutil.py:562-588)."""
+
+    def __init__(self, mpo, isite, qvec, env, status, qkey, budget=DEFAULT_BUDGET):
+        nops = mpo.nops
+        self.status = status
+        site = isite if status == 'L' else isite + 1
+        self.site = site
+        case_of = [case(mpo.opers[p], site) for p in range(nops)]
+        plans = {}
+        for p in range(nops):
+            c = case_of[p]
+            if c in plans:
+                continue
+            op = env.meta[p]
+            cop = mpo.sites[site][p]
+            t1, t2, t3 = QTensor(), QTensor(), QTensor()
+            if status == 'L':
+                t1.tensordotSYM(op, qvec, ([2], [0]))              # [w, l, n1, m]
+                t2.tensordotSYM(t1, cop, ([0, 2], [0, 3]))         # [l, m, wr, u]
+                t3.tensordotSYM(t2, t2, ([1, 2], [1, 2]))          # [l, u, l', u']
+            else:
+                rf = op.template()
+                rf.qsyms = [-op.qsyms[0], -(qkey - op.qsyms[1]), qkey - op.qsyms[2]]
+                rf.status = op.status.copy()
+                rf.status[1] = not rf.status[1]
+                rf.fromQsyms(3, rf.qsyms, rf.ndims, ifallocate=False, idlst=op.idlst)
+                if not np.array_equal(rf.idx_allowed, op.idx_allowed):
+                    raise RuntimeError('relabelled right environment has other blocks')
+                t1.tensordotSYM(qvec, rf, ([2], [2]))              # [m, n2, w, r]
+                t2.tensordotSYM(cop, t1, ([1, 3], [2, 1]))         # [wl, u, m, r]
+                t3.tensordotSYM(t2, t2, ([0, 2], [0, 2]))          # [u, r, u', r']
+            plans[c] = [t.plan.lowered() for t in (t1, t2, t3)] + [t1, t2, t3]
+        self.plans, self.case_of = plans, case_of
+        isz = [(plans[c][3].size_allowed + plans[c][4].size_allowed) * 8 for c in case_of]
+        self.chunks = _chunks(isz, budget)
+        self.cases = sorted(plans)
+        self.out_off = {}
+        tot = 0
+        for c in self.cases:
+            self.out_off[c] = tot
+            tot += plans[c][5].size_allowed
+        self.out_size = tot
+        self.stages, self.inter = [], [0, 0]
+        for ch in self.chunks:
+            offs = [0, 0]
+            parts = [[], []]
+            gram = {c: [] for c in self.cases}
+            for p in ch:
+                c = case_of[p]
+                pl = plans[c]
+                if status == 'L':
+                    parts[0].append((pl[0], env.off[p], 0, offs[0], 1.0))
+                    parts[1].append((pl[1], offs[0], mpo.off[site][p], offs[1], 1.0))
+                else:
+                    parts[0].append((pl[0], 0, env.off[p], offs[0], 1.0))
+                    parts[1].append((pl[1], mpo.off[site][p], offs[0], offs[1], 1.0))
+                gram[c].append((pl[2], offs[1], offs[1], self.out_off[c], 1.0))
+                offs[0] += pl[3].size_allowed
+                offs[1] += pl[4].size_allowed
+            progs = [Program(*_instantiate_many(parts[k])) for k in range(2)]
+            grams = [Program(*_merge_shared_many(gram[c], plans[c][5].nblks_allowed))
+                     for c in self.cases if gram[c] and plans[c][5].nblks_allowed]
+            self.stages.append((progs, grams))
+            self.inter = [max(a, b) for a, b in zip(self.inter, offs)]
+
+    def apply(self, mpo, env, cvals):
+        """Device buffer of every case's Gram blocks (case c at out_off[c])."""
+        t1 = dv.empty(self.inter[0], 0)
+        t2 = dv.empty(self.inter[1], 0)
+        out = dv.zeros(self.out_size, 0)
+        E = env.buffer(0)
+        W = mpo.buffer(self.site, 0)
+        for progs, grams in self.stages:
+            if self.status == 'L':
+                run_program(progs[0], E, cvals, t1, 0)
+                run_program(progs[1], t1, W, t2, 0)
+            else:
+                run_program(progs[0], cvals, E, t1, 0)
+                run_program(progs[1], W, t1, t2, 0)
+            for g in grams:   # each chunk adds its ops' share
+                run_program(g, t2, t2, out, 0, beta=1)
+        return out
+
+    def dense(self, values, idlst):
+        """Sum over cases of the Gram blocks as one dense matrix; idlst = the
+        index lists of the two kept legs."""
+        tot = None
+        for c in self.cases:
+            t3 = self.plans[c][5].template()
+            o = self.out_off[c]
+            t3.value = values[o:o + t3.size_allowed]
+            d = t3.toDenseTensor([idlst[0], idlst[1], idlst[0], idlst[1]])
+            tot = d if tot is None else tot + d
+        n = tot.shape[0] * tot.shape[1]
+        return tot.reshape(n, n)
+
+
+def prdm_template(dmrg, key, status):
+    """dotutil.setupQtmpPRDM (ncsite=2): the 3-leg C template and its index
+    lists; returns (qvec, idlst)."""
+    from .dot import dpt
+    qkey = np.array(key, dtype=np.float64)
+    if status == 'L':
+        qnums = [dmrg.qlst[0], dmrg.qlst[1], qkey - dpt(dmrg.qlst[2], dmrg.qlst[3])]
+    else:
+        qnums = [dpt(dmrg.qlst[0], dmrg.qlst[1]), dmrg.qlst[2], qkey - dmrg.qlst[3]]
+    qvec = QTensor([False, False, True])
+    from .qtensor import fromQnums
+    rank, qsyms, ndims, idlst = fromQnums(qnums)
+    qvec.fromQsyms(rank, qsyms, ndims, ifallocate=False)
+    qvec.idlst = [list(x) for x in idlst]
+    return qvec, idlst
+
+
+def prdm_noise(cimat, info):
+    """sum over states and ops of X_p X_p^T (the operator term of pRDMQt,
+    kernelQt.py:478-533 / 562-622), dense over the kept legs: (ldim*4)^2 for
+    'L', (4*rdim)^2 for 'R'.  cimat: float64[neig, ldim, 4, 4*rdim] ('L') or
+    [neig, ldim*4, 4, rdim] ('R'), dense product basis."""
+    dmrg, isite, flst = _info_parts(info)
+    status = info[4]
+    env = flst[0][0] if status == 'L' else flst[0][1]
+    keep = (0, 1) if status == 'L' else (1, 2)
+    out = None
+    idx = 0
+    for key, neig in dmrg.qsectors.items():
+        qvec, idlst = prdm_template(dmrg, key, status)
+        prog = PRDMProgram(dmrg.fhop, isite, qvec, env, status, np.array(key, dtype=np.float64),
+                           getattr(dmrg, 'budget', DEFAULT_BUDGET))
+        for _ in range(neig):
+            q = qvec.template()
+            q.fromDense(np.asarray(cimat[idx]), idlst)
+            idx += 1
+            vals = prog.apply(dmrg.fhop, env, dv.upload(q.value, 0))
+            d = prog.dense(dv.download(vals), [idlst[k] for k in keep])
+            out = d if out is None else out + d
+    return out
+
+
+def pRDMQt(cimat, info):
+    """Reduced density matrix of the kept side (kernelQt.py:453-626): sum
+    over states of C C^T plus dmrg.noise times the operator term
+    (prdm_noise) when dmrg.noise >= 1e-10; dense, shapes as prdm_noise."""
+    dmrg = info[0]
+    status = info[4]
+    rdm = None
+    for c in np.asarray(cimat):
+        m = c.reshape(c.shape[0] * c.shape[1], -1) if status == 'L' else \
+            c.reshape(c.shape[0], -1)
+        plain = m @ m.T if status == 'L' else m.T @ m
+        rdm = plain if rdm is None else rdm + plain
+    noise = float(getattr(dmrg, 'noise', 0.0))
+    if noise >= 1e-10:
+        rdm = rdm + noise * prdm_noise(cimat, info)
+    return rdm
+
+
 # ------------------------------------------------------------------ drop-in API
 def _info_parts(info):
     dmrg, isite, ncsite, flst, status, ifsym = info
