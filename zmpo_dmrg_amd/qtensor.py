@@ -1199,9 +1199,49 @@ def gather(dst, src, idx, dtype_dst=_lib.F64, dtype_src=_lib.F64):
     return dst
 
 
+def _scalar_contract(qt1, qt2, i1, i2):
+    """Full contraction of two block-sparse tensors (the scalar case of
+    qtensor.py:727-750): the matching allowed blocks' elements gathered into
+    two device vectors, then one fixed-order dot product."""
+    from . import pdvdson
+    r = qt1.rank
+    pair = [i2[list(i1).index(leg)] for leg in range(r)]   # leg of qt2 matching leg of qt1
+    for leg in range(r):
+        if not np.array_equal(qt1.qsyms[leg], qt2.qsyms[pair[leg]]):
+            raise ValueError('tensordot: contracted legs carry different sectors')
+    ns = [int(x) for x in qt1.nsyms]
+    k1 = np.ravel_multi_index(qt1.blk_isyms.T, ns) if qt1.nblks_allowed else np.zeros(0, np.int64)
+    k2 = np.ravel_multi_index(qt2.blk_isyms[:, pair].T, ns) if qt2.nblks_allowed else \
+        np.zeros(0, np.int64)
+    common, j1s, j2s = np.intersect1d(k1, k2, return_indices=True)
+    st1, st2 = qt1.blk_strides(), qt2.blk_strides()
+    idx1, idx2 = [], []
+    for j1, j2 in zip(j1s, j2s):
+        dims = qt1.blk_dims[j1]
+        grid = np.indices(dims).reshape(r, -1)
+        idx1.append(qt1.blk_off[j1] + (st1[j1][:, None] * grid).sum(0))
+        idx2.append(qt2.blk_off[j2] + (st2[j2][pair][:, None] * grid).sum(0))
+    n = int(sum(len(x) for x in idx1))
+    if n == 0:
+        return 0.0
+    a = gather(dv.empty(n, 0), _as_device(qt1.value, 0), dv.upload(np.concatenate(idx1)))
+    b = gather(dv.empty(n, 0), _as_device(qt2.value, 0), dv.upload(np.concatenate(idx2)))
+    return float(pdvdson._VecOps(n).dots(a.view(1, -1), 1, b)[0])
+
+
 def tensordot(qt1, qt2, axes, perm=None, dtype=_lib.F64):
-    """One-shot contraction (qtensor.py:665-827 general case) on the device."""
+    """One-shot contraction on the device (qtensor.py:665-827): the general
+    case (some legs contracted), the outer product (none: K = 1 block GEMMs)
+    and the scalar case (all legs of both: returns a float).  The output
+    keeps the legs' index lists when both inputs have them."""
+    i1, i2 = list(axes[0]), list(axes[1])
+    if len(i1) == qt1.rank and len(i2) == qt2.rank:
+        return _scalar_contract(qt1, qt2, i1, i2)
     out = QTensor()
     out.tensordotSYM(qt1, qt2, axes, perm=perm)
     out.tensordotCAL(qt1, qt2, dtype=dtype)
+    if qt1.idlst is not None and qt2.idlst is not None:
+        e1 = [k for k in range(qt1.rank) if k not in i1]
+        e2 = [k for k in range(qt2.rank) if k not in i2]
+        out.idlst = [qt1.idlst[k] for k in e1] + [qt2.idlst[k] for k in e2]
     return out
