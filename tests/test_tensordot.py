@@ -20,7 +20,7 @@ def _rand(status, dims_q, rng):
 def _cases(rng):
     a, qa = _rand([False, False, True], [4, 3, 5], rng)
     # same leg sectors for the contracted legs, opposite direction
-    b = QTensor([True, True, False])
+    b = QTensor([False, True, True])
     b.fromDenseTensor(rng.standard_normal((5, 3, 4)), [qa[2], qa[1], qa[0]])
     c, _ = _rand([False, True], [3, 3], rng)
     return a, b, c
