@@ -119,3 +119,51 @@ def fci_rdms(h, eri, nelec, sz=0.0):
     G = np.einsum('rpd,qsd->prqs', V, V)
     G -= np.einsum('qr,ps->prqs', np.eye(K), gamma)
     return float(w[0]), gamma, G
+
+
+def sector_fci_energy(h, eri, nelec, sz=0.0, nroots=1):
+    """Lowest eigenvalues in one (N, Sz) sector, built on the sector's
+    determinants only (no 4^K Fock space): E_pr restricted to the sector by
+    applying a+_p a_r to every occupation string (Jordan-Wigner signs over
+    the spin orbitals in the order 2i (alpha), 2i+1 (beta)).  Same
+    Hamiltonian as fock_hamiltonian; usable where 4^K is too large."""
+    import itertools
+    K = h.shape[0]
+    na = int(round(nelec / 2 + sz))
+    nb = int(round(nelec / 2 - sz))
+    dets = []
+    for ao in itertools.combinations(range(K), na):
+        for bo in itertools.combinations(range(K), nb):
+            occ = 0
+            for i in ao:
+                occ |= 1 << (2 * i)
+            for i in bo:
+                occ |= 1 << (2 * i + 1)
+            dets.append(occ)
+    index = {d: k for k, d in enumerate(dets)}
+    n = len(dets)
+
+    def excite(occ, p, r):
+        """a+_p a_r |occ> as (sign, new occ) or None (spin orbitals p, r)."""
+        if not (occ >> r) & 1:
+            return None
+        s = (-1) ** bin(occ & ((1 << r) - 1)).count('1')
+        occ2 = occ & ~(1 << r)
+        if (occ2 >> p) & 1:
+            return None
+        s *= (-1) ** bin(occ2 & ((1 << p) - 1)).count('1')
+        return s, occ2 | (1 << p)
+
+    E = np.zeros((K, K, n, n))
+    for col, occ in enumerate(dets):
+        for p in range(K):
+            for r in range(K):
+                for spin in (0, 1):
+                    res = excite(occ, 2 * p + spin, 2 * r + spin)
+                    if res is not None:
+                        E[p, r, index[res[1]], col] += res[0]
+    H = np.einsum('pr,prij->ij', h, E)
+    EE = np.einsum('prij,qsjk->prqsik', E, E, optimize=True)
+    H += 0.5 * np.einsum('prqs,prqsik->ik', eri, EE, optimize=True)
+    H -= 0.5 * np.einsum('prrs,psik->ik', eri, E, optimize=True)
+    return np.linalg.eigvalsh(H)[:nroots]
