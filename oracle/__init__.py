@@ -20,6 +20,9 @@ Pinning (what checks each module against something other than itself):
                               with the MPS (Hermiticity, diag == HDiag,
                               <psi|H|psi> == Davidson eigenvalue).
   davidson                 -- scipy.linalg.eigh on the explicit matrix.
+  fci.sector_fci_energy    -- fci.fci_energy (Fock-space build) at K = 4, and the
+                              BLOCK energy the reference records for its FCIDUMP
+                              case (itools/fromFCIDUMP/eBLOCK:1) to 6e-9.
   fci.fci_rdms             -- E = sum h gamma + 1/2 sum (pr|qs) Gamma, tr gamma = N,
                               and central differences of the FCI energy
                               (Hellmann-Feynman) along h and symmetric eri.
