@@ -3,9 +3,11 @@
 Counterpart of source/mpo_dmrg_class.py (mpo_dmrg fields :47-192, build
 :279-297, default :329-434) and source/mpo_dmrg_opt.py (sweep :31-95,
 checkOpers :153-193, sweep_updateSite :313-344, sweep_updateBoundary
-:347-377), with the 'Fixed2' schedule (mpo_dmrg_schedule.py:100-122), noise=0
-and one process.  Environments and the MPO live in device memory (EnvStore /
-MPOStore) instead of HDF5 files; the Qt MPS starts from an occupation
+:347-377), with the 'Fixed2' schedule (mpo_dmrg_schedule.py:100-122) and one
+process.  ``dmrg.noise`` (default 0) adds the perturbative noise term
+(pRDMQt, mpo_dmrg_kernelQt.py:453; used at mpo_dmrg_dotutil.py:313-317) to the
+decimation density matrix.  Environments and the MPO live in device memory
+(EnvStore / MPOStore) instead of HDF5 files; the Qt MPS starts from an occupation
 product state (mpo_dmrg_util.py:118-155) followed by initOps (:193-214).
 """
 import time
