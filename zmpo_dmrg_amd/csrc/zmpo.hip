@@ -1110,14 +1110,29 @@ int launch_dot(const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
 // operands make the inner loop free of bounds tests.  Rows past the group's
 // M are zeros in atile and are not written to C.
 extern __shared__ __align__(16) unsigned char chain_lds[];
+// k-steps (of 4) whose operands are loaded before their MFMAs.  FP32
+// operands take half the registers of FP64 ones, so their slice can be
+// deeper for the same register budget (more loads in flight per wave).
 #ifndef CHAIN_KS
-#define CHAIN_KS 5  // k-steps (of 4) whose operands are loaded before their MFMAs
+#define CHAIN_KS 5
 #endif
+#ifndef CHAIN_KS_F32
+#define CHAIN_KS_F32 CHAIN_KS
+#endif
+template <typename T>
+struct ChainDepth {
+  static constexpr int value = CHAIN_KS;
+};
+template <>
+struct ChainDepth<float> {
+  static constexpr int value = CHAIN_KS_F32;
+};
 
 template <typename T, int RB>
 struct ChainSlice {
-  T a[CHAIN_KS][RB];
-  T b[CHAIN_KS][2];
+  static constexpr int KS = ChainDepth<T>::value;
+  T a[KS][RB];
+  T b[KS][2];
 };
 
 // The fields of a pair that the chained stages use (uniform per wave).
@@ -1143,7 +1158,7 @@ struct ChainRows {
   int mb[RB], mr[RB];
 };
 
-// Operands of the k-steps [k0, k0 + 4*CHAIN_KS) of one pair, 32-column tile
+// Operands of the k-steps [k0, k0 + 4*KS) of one pair, 32-column tile
 // n0.  Only k-steps that start below K are loaded (uniform test); lanes past
 // K inside the last k-step read the zero padding of both operands.  Bases
 // are wave-uniform (scalar registers); each lane adds a fixed 32-bit offset.
@@ -1166,7 +1181,7 @@ __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPai
 #pragma unroll
     for (int i = 0; i < RB; ++i) ro[i] = (int64_t)rows.mb[i] * bk.a_smhi + (int64_t)rows.mr[i] * bk.a_smlo;
 #pragma unroll
-    for (int s = 0; s < CHAIN_KS; ++s) {
+    for (int s = 0; s < ChainSlice<T, RB>::KS; ++s) {
       if (k0 + 4 * s < pr.K) {
         const T* __restrict__ bs = b0 + 4 * s * pr.ldb;
         sl.b[s][0] = bs[lane_b];
@@ -1180,7 +1195,7 @@ __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPai
   }
   const T* __restrict__ a0 = src + (pr.c + k0) * LD;
 #pragma unroll
-  for (int s = 0; s < CHAIN_KS; ++s) {
+  for (int s = 0; s < ChainSlice<T, RB>::KS; ++s) {
     if (k0 + 4 * s < pr.K) {
       const T* __restrict__ bs = b0 + 4 * s * pr.ldb;
       sl.b[s][0] = bs[lane_b];
@@ -1197,7 +1212,7 @@ __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPai
 // = first column; ceil4(N) columns), else into C with the task's maps, rows
 // split by the unit's precomputed (mb, mr).  Each wave walks its tasks
 // (t0 + wave, step 4) as one flat sequence of items -- an item is a
-// 4*CHAIN_KS-deep k slice of one pair for one 32-column tile -- and loads
+// 4*KS-deep k slice of one pair for one 32-column tile -- and loads
 // the operands of item i+1 (possibly of the next pair or task) while the
 // MFMAs of item i run; the two operand slices alternate roles, so nothing
 // is copied between them.
@@ -1205,7 +1220,8 @@ template <typename T, int RB, bool TO_LDS, bool GA, bool N16>
 struct ChainWalk {
   typedef typename Acc<T>::type accv;
   static constexpr int LD = 16 * RB + 1;
-  static constexpr int KD = 4 * CHAIN_KS;
+  static constexpr int KS = ChainDepth<T>::value;
+  static constexpr int KD = 4 * KS;
   static constexpr int TW = N16 ? 16 : 32;  // column tile width
   static constexpr int NJ = N16 ? 1 : 2;    // 16-column MFMA blocks per tile
   const zmpo_task* __restrict__ tasks;
@@ -1228,7 +1244,7 @@ struct ChainWalk {
   __device__ __forceinline__ int load(ChainSlice<T, RB>& sl) {
     if (p >= tpe) return 0;
     chain_load<T, RB, GA, N16>(sl, pr, n0, k0, src, Bb, lane_a, lr, lk, Ag, blocks, rows);
-    return min(CHAIN_KS, (pr.K - k0 + 3) / 4);
+    return min(KS, (pr.K - k0 + 3) / 4);
   }
   // move the cursor to the next item; returns whether the previous item
   // closed its (task, column tile), and sets (ct, cn0, cN) to that tile
@@ -1265,7 +1281,7 @@ struct ChainWalk {
   }
   __device__ __forceinline__ void mfma(const ChainSlice<T, RB>& sl, int ks) {
 #pragma unroll
-    for (int s = 0; s < CHAIN_KS; ++s) {
+    for (int s = 0; s < KS; ++s) {
       if (s >= ks) break;  // uniform
 #pragma unroll
       for (int i = 0; i < RB; ++i)
@@ -1455,10 +1471,22 @@ int launch_chain_t(int kcols_max, int ncols_max, const zmpo_cunit* units, int64_
   const size_t lds = (size_t)((ATILE ? kcols_max : 0) + ncols_max) * (16 * RB + 1) * sizeof(T);
   if (lds > 160 * 1024) return set_error(ZMPO_ERR_ARG, "chaingemm: row tile exceeds 160 KiB of LDS");
   if (lds > 64 * 1024) {
-    int rc = check_hip(hipFuncSetAttribute((const void*)chain_gemm_kernel<T, RB, ATILE, N16>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                       "chaingemm: LDS attribute");
+    // the LDS allowance holds for the rest of the process: raise it only
+    // when a launch needs more than this device already allows, so repeated
+    // launches (and launches captured into a graph) make no attribute call
+    constexpr int MAXDEV = 64;
+    static int allowed[MAXDEV] = {0};
+    int dev = 0;
+    int rc = check_hip(hipGetDevice(&dev), "chaingemm: device");
     if (rc) return rc;
+    if (dev < 0 || dev >= MAXDEV) return set_error(ZMPO_ERR_ARG, "chaingemm: device index beyond 63");
+    if ((int)lds > allowed[dev]) {
+      rc = check_hip(hipFuncSetAttribute((const void*)chain_gemm_kernel<T, RB, ATILE, N16>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                     "chaingemm: LDS attribute");
+      if (rc) return rc;
+      allowed[dev] = (int)lds;
+    }
   }
   hipLaunchKernelGGL((chain_gemm_kernel<T, RB, ATILE, N16>), dim3((unsigned)nunits), dim3(256), lds, s,
                      units, nunits, groups, blocks, tasks, pairs, (const T*)A, (const T*)B1,

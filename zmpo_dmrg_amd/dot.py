@@ -17,6 +17,8 @@ one.  The reference's own rule depends on Python-2 dict order and an
 unstable argsort (mpo_dmrg_qparser.py:220-233, 259) and cannot be
 reproduced.
 """
+import os
+
 import numpy as np
 
 from . import device as dv
@@ -24,6 +26,13 @@ from . import kernelQt
 from .pdvdson import EigenSolver
 from .qtensor import QTensor, fromQnums
 
+
+# ZMPO_GRAPH=1: Davidson sigma applications after the first of a step replay
+# a captured HIP graph of the sigma launches (one host call instead of one per
+# launch).  Off by default: measured from K = 12, D = 128 to K = 24, D = 512
+# the sigma is bound by its kernels, not by host launch cost (replay and
+# eager launches take the same time), and each capture costs a little.
+GRAPH = os.environ.get('ZMPO_GRAPH', '0') != '0'
 
 # RDM weights (trace-normalised) at or below this are numerically zero: they
 # are ordered by QN only, and their eigenvectors get the canonical basis.
@@ -99,8 +108,27 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
         psi_lo = dv.empty(ndim0, dtype) if dtype else None
         sig_lo = dv.empty(ndim0, dtype) if dtype else None
 
+        graphs = {}
+
+        def graph_sigma(x, out, dt):
+            """From the second application in a dtype on, replay the captured
+            launches (the first one runs eagerly and allocates)."""
+            if dt in graphs:
+                graphs[dt].run(x, out)
+                return True
+            if GRAPH and dt in seen:
+                graphs[dt] = kernelQt.SigmaGraph(prog, dmrg.fhop, fL, fR, dt, warmup=False)
+                graphs[dt].run(x, out)
+                return True
+            seen.add(dt)
+            return False
+
+        seen = set()
+
         def hvec(x, out):
-            if dtype == 0:
+            if graph_sigma(x, out, dtype):
+                pass
+            elif dtype == 0:
                 prog.apply(dmrg.fhop, fL, fR, x, out, 0)
             else:
                 psi_lo.copy_(x)
@@ -110,7 +138,8 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
             dmrg.sigma_flops += prog.flops
 
         def hvec64(x, out):
-            prog.apply(dmrg.fhop, fL, fR, x, out, 0)
+            if not graph_sigma(x, out, 0):
+                prog.apply(dmrg.fhop, fL, fR, x, out, 0)
             dmrg.nsigma64 = getattr(dmrg, 'nsigma64', 0) + 1
             dmrg.sigma_flops += prog.flops
 
@@ -131,6 +160,7 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
             # the low-precision copies of this step's environments are not
             # needed again (the next step at this bond rebuilds them): drop
             # them so that only the FP64 environments stay resident
+            graphs.pop(dtype, None)
             for env in (fL, fR):
                 env._buf32 = None
             fine = EigenSolver()

@@ -276,9 +276,11 @@ class SigmaProgram:
             b[1] = dv.empty(self.inter[1], dtype)
         return b
 
-    def apply(self, mpo, lenv, renv, psi, sigma, dtype=0, fused=None):
-        """sigma = H psi (device arrays of length ndim0, same dtype)."""
+    def apply(self, mpo, lenv, renv, psi, sigma, dtype=0, fused=None, overlap=None):
+        """sigma = H psi (device arrays of length ndim0, same dtype).  overlap=False
+        keeps every launch on the current stream (as graph capture needs)."""
         fused = self.fused if fused is None else fused
+        overlap = OVERLAP if overlap is None else overlap
         t1, t2, t3 = self.buffers(dtype, mid=not fused)
         L, R = lenv.buffer(dtype), renv.buffer(dtype)
         Wi, Wj = mpo.buffer(self.isite, dtype), mpo.buffer(self.isite + 1, dtype)
@@ -287,7 +289,7 @@ class SigmaProgram:
             return sigma
         if fused and not self.fused:
             raise ValueError('SigmaProgram built without the chained stages (fused=False)')
-        if fused and OVERLAP and len(self.stages) > 1 and not isinstance(psi, np.ndarray) \
+        if fused and overlap and len(self.stages) > 1 and not isinstance(psi, np.ndarray) \
                 and getattr(psi, 'is_cuda', False):
             return self._apply_overlapped(mpo, L, R, psi, sigma, dtype, t1, t3)
         first = True
@@ -340,6 +342,63 @@ class SigmaProgram:
         if first:
             sigma.zero_()
         return sigma
+
+
+class SigmaGraph:
+    """One sigma application of a SigmaProgram captured as a HIP graph over
+    fixed input/output vectors: every launch of ``SigmaProgram.apply`` (the
+    grouped-GEMM, split-K reduce and chained dispatches) is re-issued by one
+    host call, which removes the per-launch host cost that dominates small
+    sigmas.  The graph holds references to every buffer it reads or writes
+    (environments, MPO copies, intermediates, split-K workspace), so they
+    stay allocated while it lives; replays run on the current stream.
+    The captured launches contain no memset nodes."""
+
+    def __init__(self, prog, mpo, lenv, renv, dtype=0, warmup=True):
+        """warmup=False: the caller has already run ``prog.apply`` eagerly in
+        this dtype with these environments (lazy allocations are done)."""
+        torch = dv.torch()
+        n = prog.ndim0
+        self.prog, self.dtype = prog, dtype
+        self.x = dv.zeros(n, dtype)
+        self.y = dv.zeros(n, dtype)
+        self.empty = not prog.stages or all(st[3].ntasks == 0 for st in prog.stages)
+        if self.empty:
+            self.graph = None
+            return
+        # eager run first: device tables, intermediates, converted
+        # environments and the workspace are allocated outside the capture
+        if warmup:
+            prog.apply(mpo, lenv, renv, self.x, self.y, dtype, overlap=False)
+        from .qtensor import _WORK
+        self._keep = [lenv.buffer(dtype), renv.buffer(dtype), _WORK.get(dtype),
+                      mpo.buffer(prog.isite, dtype), mpo.buffer(prog.isite + 1, dtype)]
+        if prog.fused:
+            self._keep += [mpo.padded_buffer(prog.isite, dtype),
+                           mpo.padded_buffer(prog.isite + 1, dtype)]
+        self._keep += [b for b in prog.buffers(dtype) if b is not None]
+        main = torch.cuda.current_stream()
+        cap = torch.cuda.Stream()   # default priority
+        cap.wait_stream(main)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(cap):
+            self.graph.capture_begin()
+            try:
+                prog.apply(mpo, lenv, renv, self.x, self.y, dtype, overlap=False)
+            finally:
+                self.graph.capture_end()
+        main.wait_stream(cap)
+
+    def run(self, x, out):
+        """out = H x through the captured launches (x, out: device vectors of
+        any floating dtype; converted on the copies in and out)."""
+        if self.graph is None:
+            out.zero_()
+            return out
+        self.x.copy_(x)
+        self.graph.replay()
+        out.copy_(self.y)
+        return out
 
 
 # ------------------------------------------------------------------ diagonal
