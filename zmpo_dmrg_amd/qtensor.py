@@ -636,6 +636,7 @@ class Program:
         self.flops = int(np.sum(2 * tasks['M'][rows].astype(np.int64) * tasks['N'][rows]
                                 * pairs['K'].astype(np.int64))) if self.npairs else 0
         self.kt = None    # stream kernel k-tile (None: library default)
+        self.kt_dtype = None   # stream kernel k-tile per dtype code (overrides kt)
         self.upw = None   # stream kernel units per workgroup (None: by launch size)
 
     # The device-ready arrays and the schedule choices are derived on first
@@ -811,7 +812,8 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None)
                                      beta, st))
         return
     if kernel == 'stream':
-        _lib.check(L.zmpo_streamgemm(dtype, prog.shape, sc['kt'], dv.ptr(sc['units']),
+        kt = prog.kt_dtype.get(dtype, sc['kt']) if prog.kt_dtype else sc['kt']
+        _lib.check(L.zmpo_streamgemm(dtype, prog.shape, kt, dv.ptr(sc['units']),
                                      sc['nunits'], sc['upw'], dv.ptr(tasks), dv.ptr(sc['groups']),
                                      sc['ngroups'], dv.ptr(pairs), dv.ptr(A), dv.ptr(B),
                                      dv.ptr(C), dv.ptr(work), beta, st))
