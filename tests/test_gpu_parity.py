@@ -75,6 +75,27 @@ def test_sigma_chunked_two_streams_matches_oracle(gpu, case, overlap, monkeypatc
         assert all(np.array_equal(outs[0], o) for o in outs[1:])
 
 
+def test_chain_wave_balance_changes_no_bit(gpu, case, monkeypatch):
+    """The chained launch's per-wave task order (qtensor.balance_waves) only
+    moves tasks between the four waves of a workgroup: sigma is bit-identical
+    with and without it, and equals the oracle."""
+    from zmpo_dmrg_amd import qtensor as pq
+    for isite in (1, case.K - 3):
+        qkey, oq, pt, _ = case.templates(isite)
+        v = np.random.default_rng(isite + 3).standard_normal(oq.size_allowed)
+        outs = []
+        for bal in (False, True):
+            monkeypatch.setattr(pq, 'CHAIN_BALANCE', bal)
+            prog = kq.SigmaProgram(case.mpo, isite, pt, qkey, case.L[isite - 1],
+                                   case.R[isite + 2], case.mpo.hpwts)
+            sig = dv.empty(pt.size_allowed, 0)
+            prog.apply(case.mpo, case.L[isite - 1], case.R[isite + 2], dv.upload(v, 0), sig)
+            outs.append(dv.download(sig))
+        assert np.array_equal(outs[0], outs[1]), isite
+        want = case.oracle_sigma(isite, v)
+        assert np.abs(outs[1] - want).max() < 1e-12 * max(1.0, np.abs(want).max())
+
+
 def test_sigma_fp32_within_tolerance(gpu, case):
     isite = 1
     qkey, oq, pt, _ = case.templates(isite)
