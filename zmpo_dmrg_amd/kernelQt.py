@@ -478,9 +478,16 @@ class DiagProgram:
         h1_parts, h2_rows = [], []
         ops_info = []
         pbase = 0
+        gidx = {}   # ops of one case share their environment template
+
+        def gather_index(meta):
+            if id(meta) not in gidx:
+                gidx[id(meta)] = _diag_gather_index(meta)
+            return gidx[id(meta)]
+
         for p in range(nops):
-            li, z1, ld = _diag_gather_index(lenv.meta[p])
-            ri, z3, rd = _diag_gather_index(renv.meta[p])
+            li, z1, ld = gather_index(lenv.meta[p])
+            ri, z3, rd = gather_index(renv.meta[p])
             if li is None or ri is None:
                 continue
             assert ld == ldim and rd == rdim

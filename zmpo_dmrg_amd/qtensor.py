@@ -18,6 +18,7 @@ ascending block index.  Two differences, both internal to this package:
 block order, same internal-tuple order, qtensor.py:563-637) with integer
 numpy, and lowers it to GEMM tasks (``lower_plan``).
 """
+import collections
 import os
 
 import numpy as np
@@ -49,6 +50,28 @@ def fromQnums(qnums, ifcollect=None):
     return len(qnums), [c[0] for c in cls], [c[1] for c in cls], [c[2] for c in cls]
 
 
+# Allowed-block lists by (sector counts, leg directions, sector quantum
+# numbers): a sweep step rebuilds the same tensor templates many times (about
+# five calls per distinct table), so the block rule runs once per table.
+_BLOCKS = collections.OrderedDict()
+_BLOCKS_MAX = 4096
+
+
+def _allowed_blocks(nsyms, status, qsyms):
+    """Flat indices of the allowed blocks (ascending), via zmpo_symAllowed."""
+    key = (np.asarray(nsyms, np.int64).tobytes(), bytes(int(bool(x)) for x in status),
+           tuple(q.shape[1] for q in qsyms), b''.join(q.tobytes() for q in qsyms))
+    idx = _BLOCKS.get(key)
+    if idx is None:
+        idx = np.flatnonzero(_lib.sym_allowed(nsyms, status, qsyms))
+        _BLOCKS[key] = idx
+        if len(_BLOCKS) > _BLOCKS_MAX:
+            _BLOCKS.popitem(last=False)
+    else:
+        _BLOCKS.move_to_end(key)
+    return idx.copy()
+
+
 class QTensor:
     def __init__(self, status=None, perm=None):
         self.status = None if status is None else np.array(status, dtype=bool)
@@ -75,8 +98,7 @@ class QTensor:
         self.size = int(np.prod(self.shape))
         self.nsyms = np.array([len(d) for d in self.ndims], dtype=np.int32)
         self.nblks = int(np.prod(self.nsyms))
-        mask = _lib.sym_allowed(self.nsyms, self.status, self.qsyms)
-        self.idx_allowed = np.flatnonzero(mask)
+        self.idx_allowed = _allowed_blocks(self.nsyms, self.status, self.qsyms)
         self.nblks_allowed = len(self.idx_allowed)
         ib = np.zeros(self.nblks, dtype=np.int32)
         ib[self.idx_allowed] = self.idx_allowed + 1
