@@ -516,12 +516,21 @@ def make_schedule(tasks, pairs, rb, kchunk=2048, reuse_key=None, tile=None):
     per = tm * tn * nch
     ut = np.repeat(np.arange(nt), per)
     within = np.arange(int(per.sum())) - np.repeat(np.concatenate([[0], np.cumsum(per)[:-1]]), per)
-    tilei = within // nch[ut]
-    chunk = within % nch[ut]
+    # the divisions are skipped where every divisor is 1 (no split-K, one
+    # tile column): the common case of the large launches
+    if nt and nch.max() == 1:
+        tilei, chunk = within, np.zeros(len(within), np.int64)
+    else:
+        nchu = nch[ut]
+        tilei, chunk = np.divmod(within, nchu)
     units = np.zeros(len(ut), dtype=_lib.UNIT_DTYPE)
     units['task'] = ut
-    units['m0'] = (tilei // tn[ut]) * TMs
-    units['n0'] = (tilei % tn[ut]) * TNs
+    if nt and tn.max() == 1:
+        units['m0'] = tilei * TMs
+    else:
+        tr, tc = np.divmod(tilei, tn[ut])
+        units['m0'] = tr * TMs
+        units['n0'] = tc * TNs
     gi = cfirst[ut] + chunk
     units['p_begin'] = cb[gi]
     units['p_end'] = ce[gi]
