@@ -245,17 +245,17 @@ def rdm_blkdiag(mats, classes, thresh, Dcut, side, extra=None):
         blocks.append((k, rows, rho))
     if tr < -1e-12:
         raise RuntimeError('RDM trace negative')
-    rot_cols, sigs, qsym = [], [], []
-    for k, rows, rho in blocks:
+    rot_cols, sigs, qsym, sectors = [], [], [], []
+    for b, (k, rows, rho) in enumerate(blocks):
         if tr < 1e-12:
             rho = rho + np.identity(len(rows)) / float(nrows)
         else:
             rho = rho / tr
         s2, u = np.linalg.eigh(rho)
         s2 = np.abs(s2)
-        u = canonical_null(u, s2, WZERO)
+        sectors.append([rows, u, s2, False])
         for c in range(len(rows)):
-            rot_cols.append((rows, u[:, c]))
+            rot_cols.append((b, c))
         sigs.append(s2)
         qsym += [k] * len(rows)
     sigs = np.hstack(sigs)
@@ -272,10 +272,18 @@ def rdm_blkdiag(mats, classes, thresh, Dcut, side, extra=None):
     indx = indx[:nres]
     re = sorted(range(nres), key=lambda x: qsym[indx[x]])
     indx = indx[re]
+    # the null-space basis of a sector (canonical_null) only matters when
+    # one of its zero-weight vectors is kept: build it for those sectors only
+    for i in indx:
+        b, c = rot_cols[i]
+        sec = sectors[b]
+        if not sec[3] and sec[2][c] <= WZERO:
+            sec[1] = canonical_null(sec[1], sec[2], WZERO)
+            sec[3] = True
     rot = np.zeros((nrows, nres))
     for c, i in enumerate(indx):
-        rows, vec = rot_cols[i]
-        rot[rows, c] = vec
+        b, col = rot_cols[i]
+        rot[sectors[b][0], c] = sectors[b][1][:, col]
     sig_kept = sigs[indx]
     return 1.0 - np.sum(sig_kept), np.array([qsym[i] for i in indx]), rot, sig_kept
 
