@@ -74,3 +74,21 @@ def test_sweeps_with_and_without_graphs_agree(gpu, monkeypatch):
             res[on, dtype] = (e, d.nsigma, getattr(d, 'nsigma64', 0))
     for dtype in (0, 1):
         assert res[True, dtype] == res[False, dtype], (dtype, res[True, dtype], res[False, dtype])
+
+
+def test_sweeps_with_plan_cache_agree(gpu, monkeypatch):
+    """Sigma programs reused across sweeps for an unchanged block structure
+    (ZMPO_PLAN_CACHE) give the same energies and sigma counts as programs
+    rebuilt every step."""
+    K = 6
+    i1, i2 = gen_integrals(K, seed=7)
+    res = {}
+    for on in (False, True):
+        monkeypatch.setattr(kq, 'PLAN_CACHE', on)
+        print('[plan-cache] on=%s' % on, flush=True)
+        d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=16, tol=1e-10)
+        e = pdmrg.run(d, 4)
+        res[on] = (e, d.nsigma)
+        if on:
+            assert getattr(d, 'plan_hits', 0) > 0
+    assert res[True] == res[False], res

@@ -20,12 +20,14 @@ Device layouts (leg storage order inside each block):
   environments [bra, mpo, ket]; top1 [b, w, n1, n2, r]; top3 [b, wr2, r, u1, u2];
   every other tensor in the reference's row-major order.
 """
+import collections
+import hashlib
+import os
+
 import numpy as np
 
 from . import _lib
 from . import device as dv
-import os
-
 from .qtensor import ChainProgram, Program, QTensor, chain_structure, gather, run_chain, run_program
 
 ENV_PERM = (1, 0, 2)
@@ -53,6 +55,10 @@ STAGE4_KT = {0: 16, 1: 32}
 # (measured at K = 36, D = 1000: L x psi 15.6 ms vs 16.6 for 64x16, x R 13.9
 # ms vs 17.5 for zmpo_dotgemm)
 SIGMA_SHAPES = [os.environ.get('ZMPO_SHAPE_S1', '2'), os.environ.get('ZMPO_SHAPE_S4', '2')]
+# ZMPO_PLAN_CACHE=1: keep each site's sigma program (plan tables only; the
+# intermediates are released between steps) and reuse it when a later sweep
+# meets the same block structure at that site
+PLAN_CACHE = os.environ.get('ZMPO_PLAN_CACHE', '0') != '0'
 
 
 def case(porb, isite, ncsite=1):
@@ -832,12 +838,40 @@ def _info_parts(info):
     return dmrg, isite, flst
 
 
+def _structure_key(isite, qtmp, fL, fR):
+    """Bytes that fix a sigma program's plans: the site, the psi template's
+    legs, every distinct environment template's legs and the per-operator
+    offsets of both environments."""
+    parts = [np.int64(isite).tobytes()]
+    for t in [qtmp] + list({id(m): m for m in list(fL.meta) + list(fR.meta)}.values()):
+        parts.append(np.int64(len(t.qsyms)).tobytes())
+        for q, d in zip(t.qsyms, t.ndims):
+            parts += [np.ascontiguousarray(q).tobytes(), np.ascontiguousarray(d).tobytes(), b'|']
+    parts += [np.asarray(fL.off, np.int64).tobytes(), b'/', np.asarray(fR.off, np.int64).tobytes()]
+    return hashlib.sha1(b''.join(parts)).hexdigest()
+
+
 def _sigma_program(dmrg, isite, fL, fR):
     key = (isite, id(dmrg.qtmp), id(fL), id(fR))
     cache = getattr(dmrg, '_sigma_cache', None)
     if cache is None or cache[0] != key:
-        prog = SigmaProgram(dmrg.fhop, isite, dmrg.qtmp, dmrg.qkey, fL, fR, dmrg.hpwts,
-                            getattr(dmrg, 'budget', DEFAULT_BUDGET))
+        if cache is not None:
+            cache[1]._buffers.clear()   # intermediates are re-allocated on use
+        prog = None
+        if PLAN_CACHE:
+            store = dmrg.__dict__.setdefault('_plan_cache', collections.OrderedDict())
+            skey = _structure_key(isite, dmrg.qtmp, fL, fR)
+            prog = store.get(skey)
+            if prog is not None:
+                store.move_to_end(skey)
+                dmrg.plan_hits = getattr(dmrg, 'plan_hits', 0) + 1
+        if prog is None:
+            prog = SigmaProgram(dmrg.fhop, isite, dmrg.qtmp, dmrg.qkey, fL, fR, dmrg.hpwts,
+                                getattr(dmrg, 'budget', DEFAULT_BUDGET))
+            if PLAN_CACHE:
+                store[skey] = prog
+                while len(store) > 2 * dmrg.nsite:
+                    store.popitem(last=False)
         dmrg._sigma_cache = (key, prog)
     return dmrg._sigma_cache[1]
 
