@@ -1165,21 +1165,26 @@ class ChainProgram:
         pfl = 2 * t['M'][prow].astype(np.int64) * t['N'][prow] * p['K'].astype(np.int64)
         tfl = np.bincount(prow, weights=pfl, minlength=len(t)) if len(p) else np.zeros(len(t))
         self.instance_cases = [c for c, *_ in instances]
+        templ = {}   # per case: unit records without the operator's bases, flops
         for c, a0, b10, b20, c0 in instances:
             g = structures[c][0]
             if len(g) == 0:
                 continue
-            ntile = np.maximum((g['M'].astype(np.int64) + tm - 1) // tm, 1)
-            gi = np.repeat(np.arange(len(g)), ntile)
-            m0 = (np.arange(int(ntile.sum())) - np.repeat(np.cumsum(ntile) - ntile, ntile)) * tm
-            u = np.zeros(len(gi), _lib.CUNIT_DTYPE)
+            if c not in templ:
+                ntile = np.maximum((g['M'].astype(np.int64) + tm - 1) // tm, 1)
+                gi = np.repeat(np.arange(len(g)), ntile)
+                m0 = (np.arange(int(ntile.sum())) - np.repeat(np.cumsum(ntile) - ntile, ntile)) * tm
+                u = np.zeros(len(gi), _lib.CUNIT_DTYPE)
+                u['group'] = gi + gbase[c]
+                u['m0'] = m0
+                gl = gbase[c]
+                templ[c] = (u, int(tfl[self.groups_host['t1_begin'][gl]:
+                                       self.groups_host['t2_end'][gl + len(g) - 1]].sum()))
+            u0, fl = templ[c]
+            u = u0.copy()
             u['a_base'], u['b1_base'], u['b2_base'], u['c_base'] = a0, b10, b20, c0
-            u['group'] = gi + gbase[c]
-            u['m0'] = m0
             uparts.append(u)
-            gl = gbase[c]
-            self.flops += int(tfl[self.groups_host['t1_begin'][gl]:
-                                  self.groups_host['t2_end'][gl + len(g) - 1]].sum())
+            self.flops += fl
         self.units_host = np.concatenate(uparts) if uparts else np.zeros(0, _lib.CUNIT_DTYPE)
         self.nunits = len(self.units_host)
         # LDS classes, by the columns a group keeps in LDS: the intermediate,
