@@ -202,3 +202,64 @@ def test_chain_program_matches_two_stages(rb):
         emulate_chain(ch, A, Pi, Pj, got)
         assert np.abs(got - want).max() < 1e-11 * max(1.0, np.abs(want).max())
         assert ch.flops == st[1].flops + st[2].flops
+
+
+def test_balance_waves_keeps_ranges_and_evens_waves():
+    """balance_waves permutes tasks only inside each row group's stage range;
+    on these seeded synthetic ranges its most loaded wave is never above the
+    original order's and within 4/3 of a brute-force search over all
+    stride-4 placements (ranges of at most 7 tasks)."""
+    import itertools
+    rng = np.random.default_rng(3)
+    # synthetic row groups: ranges of 1..9 tasks, pairs with K 1..40
+    lens = rng.integers(1, 10, size=40)
+    nt = int(lens.sum())
+    groups = np.zeros(20, pq._lib.CGROUP_DTYPE)
+    off = np.concatenate([[0], np.cumsum(lens)])
+    groups['t1_begin'], groups['t1_end'] = off[0:40:2], off[1:41:2]
+    groups['t2_begin'], groups['t2_end'] = off[1:41:2], off[2:41:2]
+    tasks = np.zeros(nt, pq._lib.TASK_DTYPE)
+    npair = rng.integers(1, 4, size=nt)
+    tasks['p_begin'] = np.concatenate([[0], np.cumsum(npair)[:-1]])
+    tasks['p_end'] = np.cumsum(npair)
+    tasks['N'] = rng.integers(1, 70, size=nt)
+    tasks['c_off'] = np.arange(nt) * 1000
+    pairs = np.zeros(int(npair.sum()), pq._lib.PAIR_DTYPE)
+    pairs['K'] = rng.integers(1, 41, size=len(pairs))
+    perm = pq.balance_waves(groups, tasks, pairs)
+    assert np.array_equal(np.sort(perm), np.arange(nt))
+    ks = (pairs['K'].astype(np.int64) + 3) // 4
+    cum = np.concatenate([[0], np.cumsum(ks)])
+    cost = (cum[tasks['p_end']] - cum[tasks['p_begin']] + 1) * ((tasks['N'].astype(np.int64) + 31) // 32)
+
+    def wave_max(c):
+        return max(c[w::4].sum() for w in range(4))
+
+    for lo, hi in zip(off[:-1], off[1:]):
+        assert sorted(perm[lo:hi]) == list(range(lo, hi))          # stays in its range
+        assert wave_max(cost[perm[lo:hi]]) <= wave_max(cost[lo:hi])
+        if hi - lo <= 7:
+            best = min(wave_max(cost[list(q)]) for q in itertools.permutations(range(lo, hi)))
+            assert wave_max(cost[perm[lo:hi]]) <= best * 4 / 3 + 1e-9
+
+
+def test_allowed_block_cache_matches_rule_and_is_private():
+    """fromQsyms' cached allowed-block lists equal the block rule's result and
+    each tensor gets its own copy."""
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        rank = int(rng.integers(2, 5))
+        nsyms = rng.integers(1, 5, size=rank)
+        qsyms = [rng.integers(-2, 3, size=(n, 2)).astype(np.float64) for n in nsyms]
+        ndims = [np.ones(n, np.int64) for n in nsyms]
+        status = [bool(x) for x in rng.integers(0, 2, size=rank)]
+        a = pq.QTensor(status)
+        a.fromQsyms(rank, qsyms, ndims)
+        b = pq.QTensor(status)
+        b.fromQsyms(rank, qsyms, ndims)                             # cache hit
+        want = np.flatnonzero(pq._lib.sym_allowed(np.array(nsyms, np.int32), status,
+                                                  [q.reshape(len(q), -1) for q in qsyms]))
+        assert np.array_equal(a.idx_allowed, want) and np.array_equal(b.idx_allowed, want)
+        if len(a.idx_allowed):
+            a.idx_allowed[0] = -7
+            assert b.idx_allowed[0] != -7
