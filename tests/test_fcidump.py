@@ -69,3 +69,24 @@ def test_device_dmrg_on_fcidump_reaches_block_energy(gpu):
     print('[fcidump] DMRG %.10f FCI %.10f BLOCK %.10f' % (etot, efci, float(d['eblock'])), flush=True)
     assert abs(etot - efci) < 1e-8
     assert abs(etot - float(d['eblock'])) < 5e-8
+
+
+def test_orbital_energy_lines_are_skipped_and_bad_rows_rejected(tmp_path):
+    """'e i 0 0 0' orbital-energy lines leave h untouched; a row such as
+    'v i j k 0' is malformed and raises."""
+    h, eri = gen_spatial(3, seed=2)
+    path = str(tmp_path / 'FCIDUMP')
+    fcidump.write(path, h, eri, 0.5, 4, 0)
+    with open(path, 'a') as f:
+        f.write('-1.25  1  0  0  0\n-0.75  3  0  0  0\n')
+    r = fcidump.read(path)
+    assert np.array_equal(r['h'], fcidump.read(path)['h'])
+    ref = str(tmp_path / 'FCIDUMP0')
+    fcidump.write(ref, h, eri, 0.5, 4, 0)
+    r0 = fcidump.read(ref)
+    assert np.array_equal(r['h'], r0['h']) and np.array_equal(r['eri'], r0['eri'])
+    assert r['ecore'] == 0.5
+    with open(path, 'a') as f:
+        f.write('0.1  1  2  3  0\n')
+    with pytest.raises(ValueError):
+        fcidump.read(path)

@@ -206,9 +206,11 @@ def test_chain_program_matches_two_stages(rb):
 
 def test_balance_waves_keeps_ranges_and_evens_waves():
     """balance_waves permutes tasks only inside each row group's stage range;
-    on these seeded synthetic ranges its most loaded wave is never above the
-    original order's and within 4/3 of a brute-force search over all
-    stride-4 placements (ranges of at most 7 tasks)."""
+    on these seeded synthetic ranges, cost-sorted as chain_structure hands
+    them over, its most loaded wave is never above the round-robin of that
+    sorted order (what the kernel would otherwise run) and within 4/3 of a
+    brute-force search over all stride-4 placements (ranges of at most 7
+    tasks)."""
     import itertools
     rng = np.random.default_rng(3)
     # synthetic row groups: ranges of 1..9 tasks, pairs with K 1..40
@@ -226,6 +228,13 @@ def test_balance_waves_keeps_ranges_and_evens_waves():
     tasks['c_off'] = np.arange(nt) * 1000
     pairs = np.zeros(int(npair.sum()), pq._lib.PAIR_DTYPE)
     pairs['K'] = rng.integers(1, 41, size=len(pairs))
+    # the production input is already cost-sorted inside each range by
+    # chain_structure (k-steps x 32-column tiles, descending, stable)
+    prow = np.repeat(np.arange(nt), npair)
+    kp = np.bincount(prow, weights=(pairs['K'].astype(np.int64) + 3) // 4, minlength=nt)
+    sort_cost = kp * ((tasks['N'].astype(np.int64) + 31) // 32)
+    seg = np.repeat(np.arange(len(lens)), lens)
+    tasks = tasks[np.lexsort((np.arange(nt), -sort_cost, seg))]
     perm = pq.balance_waves(groups, tasks, pairs)
     assert np.array_equal(np.sort(perm), np.arange(nt))
     ks = (pairs['K'].astype(np.int64) + 3) // 4

@@ -4,6 +4,8 @@ Counterpart of itools/fromFCIDUMP/tools_io.py:19-52 (loadERIs): the header
 gives NORB, NELEC, MS2; every following line is "value i j k l" (1-based)
 with (ij|kl) for four non-zero indices (8-fold symmetric, restored here),
 h_ij for k = l = 0 (symmetric) and the core energy for i = j = k = l = 0.
+Orbital-energy lines (i > 0, j = k = l = 0) are skipped; any other index
+pattern with zeros raises ValueError.
 Returns spatial h[K,K] and chemists' eri[K,K,K,K]; integrals.spin_orbital
 maps them to the MPO convention (tools_itrf.py:22-35).
 """
@@ -38,9 +40,18 @@ def read(path, ordering=None):
     h = np.zeros((n, n))
     eri = np.zeros((n, n, n, n))
     ecore = 0.0
-    two = (idx[:, 2] >= 0) & (idx[:, 3] >= 0)
-    one = ~two & (idx[:, 0] >= 0)
-    core = ~two & (idx[:, 0] < 0)
+    pos = idx >= 0
+    two = pos.all(axis=1)
+    one = pos[:, 0] & pos[:, 1] & ~pos[:, 2] & ~pos[:, 3]
+    core = ~pos.any(axis=1)
+    # 'e i 0 0 0' lines carry orbital energies, which some producers write
+    # after the integrals; they are not part of the Hamiltonian (the
+    # reference's loader would store them as h[i, K-1], tools_io.py:36-40)
+    orbe = pos[:, 0] & ~pos[:, 1] & ~pos[:, 2] & ~pos[:, 3]
+    bad = ~(two | one | core | orbe)
+    if np.any(bad):
+        raise ValueError('FCIDUMP line with an invalid index pattern: %s' % (
+            ' '.join('%d' % x for x in idx[np.flatnonzero(bad)[0]] + 1),))
     if np.any(core):
         ecore = float(val[core][-1])
     i, j = idx[one, 0], idx[one, 1]

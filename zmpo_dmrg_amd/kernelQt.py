@@ -278,6 +278,31 @@ class SigmaProgram:
         self.flops = int(self.flops_stage.sum())
         self._buffers = {}
 
+    def device_refs(self):
+        """The uploaded plan tables and schedules the launches read (a
+        captured graph keeps them alive)."""
+        out = []
+        for st in self.stages:
+            for pg in st:
+                if pg._dev is not None:
+                    out += list(pg._dev)
+                for sc in (pg._sched or {}).values():
+                    out += [sc[k] for k in ('units', 'groups') if k in sc]
+        for ch in self.chains:
+            if ch is not None and ch._dev is not None:
+                out += list(ch._dev[0]) + list(ch._dev[1])
+        return out
+
+    def release_device(self):
+        """Drop the uploaded plan tables of every launch (host arrays stay;
+        the next apply uploads them again)."""
+        for st in self.stages:
+            for pg in st:
+                pg.release_device()
+        for ch in self.chains:
+            if ch is not None:
+                ch._dev = None
+
     def buffers(self, dtype, mid=None):
         """Intermediate buffers [top1, top2, top3]; top2 is allocated only for
         the unfused path (mid=True, default: not self.fused)."""
@@ -391,6 +416,7 @@ class SigmaGraph:
             self._keep += [mpo.padded_buffer(prog.isite, dtype),
                            mpo.padded_buffer(prog.isite + 1, dtype)]
         self._keep += [b for b in prog.buffers(dtype) if b is not None]
+        self._keep += prog.device_refs()
         main = torch.cuda.current_stream()
         cap = torch.cuda.Stream()   # default priority
         cap.wait_stream(main)
@@ -838,16 +864,26 @@ def _info_parts(info):
     return dmrg, isite, flst
 
 
-def _structure_key(isite, qtmp, fL, fR):
-    """Bytes that fix a sigma program's plans: the site, the psi template's
-    legs, every distinct environment template's legs and the per-operator
-    offsets of both environments."""
-    parts = [np.int64(isite).tobytes()]
-    for t in [qtmp] + list({id(m): m for m in list(fL.meta) + list(fR.meta)}.values()):
+def _structure_key(dmrg, isite, qtmp, fL, fR):
+    """Bytes that fix a sigma program's plans: the MPO store and its weights,
+    the intermediate budget and the launch switches, the site, the psi
+    template's legs, every distinct environment template's legs, which
+    template each operator uses, and the per-operator offsets of both
+    environments."""
+    budget = getattr(dmrg, 'budget', DEFAULT_BUDGET)
+    parts = [np.array([id(dmrg.fhop), budget, int(FUSED), SIGMA_CHUNKS, CHAIN_RB, isite],
+                      np.int64).tobytes(),
+             np.ascontiguousarray(dmrg.hpwts, np.float64).tobytes()]
+    tid = {}
+    for m in list(fL.meta) + list(fR.meta):
+        tid.setdefault(id(m), (len(tid), m))
+    for t in [qtmp] + [m for _, m in tid.values()]:
         parts.append(np.int64(len(t.qsyms)).tobytes())
         for q, d in zip(t.qsyms, t.ndims):
             parts += [np.ascontiguousarray(q).tobytes(), np.ascontiguousarray(d).tobytes(), b'|']
-    parts += [np.asarray(fL.off, np.int64).tobytes(), b'/', np.asarray(fR.off, np.int64).tobytes()]
+    parts += [np.array([tid[id(m)][0] for m in fL.meta], np.int64).tobytes(), b'#',
+              np.array([tid[id(m)][0] for m in fR.meta], np.int64).tobytes(), b'#',
+              np.asarray(fL.off, np.int64).tobytes(), b'/', np.asarray(fR.off, np.int64).tobytes()]
     return hashlib.sha1(b''.join(parts)).hexdigest()
 
 
@@ -856,11 +892,14 @@ def _sigma_program(dmrg, isite, fL, fR):
     cache = getattr(dmrg, '_sigma_cache', None)
     if cache is None or cache[0] != key:
         if cache is not None:
-            cache[1]._buffers.clear()   # intermediates are re-allocated on use
+            # intermediates and the uploaded plan tables are re-created on
+            # use: a cached program keeps only host arrays
+            cache[1]._buffers.clear()
+            cache[1].release_device()
         prog = None
         if PLAN_CACHE:
             store = dmrg.__dict__.setdefault('_plan_cache', collections.OrderedDict())
-            skey = _structure_key(isite, dmrg.qtmp, fL, fR)
+            skey = _structure_key(dmrg, isite, dmrg.qtmp, fL, fR)
             prog = store.get(skey)
             if prog is not None:
                 store.move_to_end(skey)

@@ -765,30 +765,42 @@ class Program:
             self._dev = (dv.upload_bytes(self.tasks_host), dv.upload_bytes(self.pairs_host))
         return self._dev
 
+    def release_device(self):
+        """Free the uploaded tables and schedules; host arrays are kept and
+        uploaded again on the next launch."""
+        self._dev = None
+        for sc in (self._sched or {}).values():
+            sc.pop('units', None)
+            sc.pop('groups', None)
+
     def schedule(self, kernel='wave'):
         """Device units / split-K groups for the wave or shaped kernel (built once each)."""
         if self._sched is None:
             self._sched = {}
         if kernel == 'skinny' and kernel not in self._sched:
             u = self.skinny_units(SKINNY_RB)
-            self._sched[kernel] = {'units': dv.upload_bytes(u), 'nunits': len(u), 'rb': SKINNY_RB}
+            self._sched[kernel] = {'units_dev_host': u, 'nunits': len(u), 'rb': SKINNY_RB}
         if kernel not in self._sched:
             tile = None if kernel == 'wave' else (32, 32) if kernel == 'dot' else SHAPES[self.shape]
             units, groups, nslots = make_schedule(self.tasks_host, self.pairs_host, self.rb,
                                                   reuse_key=self.reuse_key, tile=tile)
             size = self.rb * 256 if kernel == 'wave' else tile[0] * tile[1]
-            sc = {'nunits': len(units), 'groups': dv.upload_bytes(groups),
-                  'ngroups': len(groups), 'nslots': nslots, 'slot_size': size,
-                  'units_host': units, 'groups_host': groups}
+            sc = {'nunits': len(units), 'ngroups': len(groups), 'nslots': nslots,
+                  'slot_size': size, 'units_host': units, 'groups_host': groups}
             if kernel == 'stream':
                 su = stream_units(self.tasks_host, units)
-                sc['units'] = dv.upload_bytes(su)
+                sc['units_dev_host'] = su
                 sc['kt'] = self.kt or STREAM_KT
                 sc['upw'] = self.upw or STREAM_UPW or stream_upw(len(su))
             else:
-                sc['units'] = dv.upload_bytes(units)
+                sc['units_dev_host'] = units
             self._sched[kernel] = sc
-        return self._sched[kernel]
+        sc = self._sched[kernel]
+        if 'units' not in sc:
+            sc['units'] = dv.upload_bytes(sc['units_dev_host'])
+        if 'groups' not in sc and 'groups_host' in sc:
+            sc['groups'] = dv.upload_bytes(sc['groups_host'])
+        return sc
 
 
 _WORK = {}
