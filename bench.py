@@ -28,7 +28,10 @@ def log(msg):
     print('[bench] ' + msg, file=sys.stderr, flush=True)
 
 
-def build_workload(K, D, seed, dtype, budget, fused=True):
+def build_workload(K, D, seed, dtype, budget, fused=True, extra=None):
+    """Middle-bond sigma workload: (mpo, isite, st, L, R, prog).  ``extra``
+    (a dict) also receives the environments one site before L and R with
+    the site tensors that renormalise them into L and R."""
     from zmpo_dmrg_amd import device as dv
     from zmpo_dmrg_amd import kernelQt as kq
     from zmpo_dmrg_amd import mpo as pmpo
@@ -48,6 +51,8 @@ def build_workload(K, D, seed, dtype, budget, fused=True):
     for i in range(isite):
         dv.sync()
         t = time.time()
+        if extra is not None and i == isite - 1:
+            extra.update(Lprev=env, lsite=ls[i], lidx=i)
         env, prog = kq.renorm(mpo, i, ls[i], env, 'L')
         dv.sync()
         ren_time += time.time() - t
@@ -57,6 +62,8 @@ def build_workload(K, D, seed, dtype, budget, fused=True):
     for i in range(K - 1, isite + 1, -1):
         dv.sync()
         t = time.time()
+        if extra is not None and i == isite + 2:
+            extra.update(Rprev=env, rsite=rs[i], ridx=i)
         env, prog = kq.renorm(mpo, i, rs[i], env, 'R')
         dv.sync()
         ren_time += time.time() - t
@@ -140,45 +147,58 @@ def traffic_per_launch(args, launch):
     return rec.get('bytes_per_launch')
 
 
-def cpu_baseline(mpo, isite, st, L, R, nops_sample, psi_host):
-    """Oracle HVecQt (numpy, the reference's per-block loop) on a subset of ops."""
-    from oracle import kernels as ok
+def to_oracle(q):
+    """An oracle QTensor with q's tables and values (host copy; values from
+    the device layout into the reference's row-major blocks)."""
     from oracle.qtensor import QTensor as OQ
+    o = OQ(q.status)
+    o.fromQsyms(q.rank, q.qsyms, q.ndims, ifallocate=False, idlst=q.idlst)
+    o.value = np.zeros(o.size_allowed)
+    v = q._host_value()
+    inv = np.argsort(q.perm)
+    for j in range(q.nblks_allowed):
+        dm = q.blk_dims[j]
+        blk = v[q.blk_off[j]:q.blk_off[j] + q.blk_size[j]].reshape(dm[list(q.perm)]).transpose(inv)
+        o.value[o.ioff_allowed[q.idx_allowed[j]]:o.ioff_allowed[q.idx_allowed[j]] + blk.size] = blk.reshape(-1)
+    return o
 
-    def conv(q):
-        o = OQ(q.status)
-        o.fromQsyms(q.rank, q.qsyms, q.ndims, ifallocate=False, idlst=q.idlst)
-        o.value = np.zeros(o.size_allowed)
-        # values from the device layout into the reference's row-major blocks
-        v = q._host_value()
-        inv = np.argsort(q.perm)
-        for j in range(q.nblks_allowed):
-            dm = q.blk_dims[j]
-            blk = v[q.blk_off[j]:q.blk_off[j] + q.blk_size[j]].reshape(dm[list(q.perm)]).transpose(inv)
-            o.value[o.ioff_allowed[q.idx_allowed[j]]:o.ioff_allowed[q.idx_allowed[j]] + blk.size] = blk.reshape(-1)
-        return o
 
-    sample = list(range(0, mpo.nops, max(1, mpo.nops // nops_sample)))[:nops_sample]
-
+def oracle_state(mpo, sites, sample):
+    """The oracle's operator state restricted to the operators in sample
+    (weights 1) with the MPO site tensors of ``sites``."""
     class S:
         pass
     s = S()
     s.nops = len(sample)
     s.opers = [mpo.opers[p] for p in sample]
     s.hpwts = np.ones(len(sample))
-    s.mpo = {isite: [conv(mpo.sites[isite][p]) for p in sample],
-             isite + 1: [conv(mpo.sites[isite + 1][p]) for p in sample]}
-    lenv = [conv(L.op_tensor(p)) for p in sample]
-    renv = [conv(R.op_tensor(p)) for p in sample]
+    s.mpo = {i: [to_oracle(mpo.sites[i][p]) for p in sample] for i in sites}
+    return s
+
+
+def oracle_sigma(mpo, isite, st, L, R, sample, psi_host):
+    """Oracle HVecQt (numpy, the reference's per-block loop) summed over the
+    operators in sample.  Returns (sigma, algorithmic flops, seconds)."""
+    from oracle import kernels as ok
+    from oracle.qtensor import QTensor as OQ
+    s = oracle_state(mpo, (isite, isite + 1), sample)
+    lenv = [to_oracle(L.op_tensor(p)) for p in sample]
+    renv = [to_oracle(R.op_tensor(p)) for p in sample]
     qt = OQ(st.qtmp.status)
     qt.fromQsyms(st.qtmp.rank, st.qtmp.qsyms, st.qtmp.ndims)
     s.qtmp = qt
     s.qkey = st.qkey
     stats = {}
     t = time.time()
-    ok.HVecQt(psi_host, s, lenv, renv, isite, stats)
+    sig = ok.HVecQt(psi_host, s, lenv, renv, isite, stats)
     dt = time.time() - t
-    fl = sum(stats.values())
+    return sig, sum(stats.values()), dt
+
+
+def cpu_baseline(mpo, isite, st, L, R, nops_sample, psi_host):
+    """Oracle HVecQt (numpy, the reference's per-block loop) on a subset of ops."""
+    sample = list(range(0, mpo.nops, max(1, mpo.nops // nops_sample)))[:nops_sample]
+    _, fl, dt = oracle_sigma(mpo, isite, st, L, R, sample, psi_host)
     return fl, dt, sample
 
 

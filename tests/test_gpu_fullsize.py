@@ -1,6 +1,9 @@
 """The sigma and diagonal launches at the full K=36, D=1000 size (72 operators,
-ndim0 = 278932; the oracle cannot run this size), checked through properties
-that hold at any size:
+ndim0 = 278932).  Against the oracle on one operator of each class (the
+oracle cannot run all 72 in reasonable time): sigma and the diagonal through
+the drop-in HVecQt / HDiagQt, and every block of the renormalised left and
+right environments of those operators.  Through properties that hold at any
+size, on all operators:
 
 * H_eff is symmetric: u.(H v) = v.(H u);
 * sigma is linear: H(a u + b v) = a H u + b H v;
@@ -17,12 +20,38 @@ from zmpo_dmrg_amd import kernelQt as kq
 pytestmark = pytest.mark.gpu
 
 
+EXTRA = {}
+
+
 @pytest.fixture(scope='module')
 def full(gpu):
     import bench
-    mpo, isite, st, L, R, prog = bench.build_workload(36, 1000, 0, 0, 96 * 2 ** 30, fused=True)
+    print('[fullsize] building K=36 D=1000', flush=True)
+    mpo, isite, st, L, R, prog = bench.build_workload(36, 1000, 0, 0, 96 * 2 ** 30, fused=True,
+                                                      extra=EXTRA)
     print('[fullsize] K=36 D=1000 ndim0=%d' % st.qtmp.size_allowed, flush=True)
-    return mpo, isite, st, L, R, prog
+    yield mpo, isite, st, L, R, prog
+    EXTRA.clear()
+
+
+def test_fullsize_sigma_and_diagonal_match_oracle_on_each_case(full):
+    from fullsize_util import check_sigma_and_diag
+    mpo, isite, st, L, R, prog = full
+    serr, derr, sample = check_sigma_and_diag(mpo, isite, st, L, R, 96 * 2 ** 30)
+    assert len(sample) == 8
+    assert serr < 1e-12 and derr < 1e-12
+
+
+@pytest.mark.parametrize('side', ['L', 'R'])
+def test_fullsize_renorm_matches_oracle(full, side):
+    from fullsize_util import check_renorm, one_op_per_case
+    mpo, isite, st, L, R, prog = full
+    if side == 'L':
+        prev, site, idx, new = EXTRA['Lprev'], EXTRA['lsite'], EXTRA['lidx'], L
+    else:
+        prev, site, idx, new = EXTRA['Rprev'], EXTRA['rsite'], EXTRA['ridx'], R
+    sample = one_op_per_case(mpo, idx, 1)[:3]
+    assert check_renorm(mpo, prev, site, idx, new, side, sample) < 1e-12
 
 
 def _sigma(full, v, dtype=0):
