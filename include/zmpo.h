@@ -252,6 +252,25 @@ int zmpo_chaingemm16(int dtype, int rb, int atile, int kcols_max, int ncols_max,
                      const void* A, const void* B1, const void* B2, void* C, int beta,
                      void* stream);
 
+/* zmpo_streamgemm without a workspace argument: the split-K partial tiles
+ * go to a file-scope device array of ZMPO_SCRATCH_BYTES (64 MiB) that each
+ * GPU holds until the process ends, so the call allocates nothing (no
+ * hipMalloc, no library allocation).  No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  work_elems = the
+ * elements the schedule's split-K slots use (slots x TM x TN, as the work
+ * buffer of zmpo_streamgemm); more than the scratch holds is rejected with
+ * ZMPO_ERR_ARG.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams, other threads) race.  Results are
+ * bit-identical to zmpo_streamgemm. */
+#define ZMPO_SCRATCH_BYTES (64ull << 20)
+int64_t zmpo_scratch_bytes(void);
+int zmpo_streamgemm_static(int dtype, int shape, int kt, const zmpo_sunit* units,
+                           int64_t nunits, int upw, const zmpo_task* tasks,
+                           const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
+                           const void* A, const void* B, void* C, int64_t work_elems, int beta,
+                           void* stream);
+
 /* out[j] = sum_i X[j*ldx + i] * y[i] for j < nx (device out, float64;
  * work holds zmpo_dots_workspace(nx) doubles).  Fixed reduction order. */
 int zmpo_dots(int dtype, const void* X, int64_t ldx, int nx, const void* y, int64_t n,

@@ -22,7 +22,7 @@ def _random_qsyms(rng, rank):
 def test_library_exports_every_header_symbol():
     lib = L.lib()
     hdr = open(os.path.join(ROOT, 'include', 'zmpo.h')).read()
-    names = set(re.findall(r'^(?:int|const char\*)\s+(zmpo_\w+)\s*\(', hdr, re.M))
+    names = set(re.findall(r'^(?:int64_t|int|const char\*)\s+(zmpo_\w+)\s*\(', hdr, re.M))
     assert names, 'no declarations parsed'
     for n in sorted(names):
         assert hasattr(lib, n), n

@@ -16,6 +16,8 @@ LIB_PATH = os.environ.get('ZMPO_LIB') or os.path.join(_HERE, 'libzmpo.so')
 
 F64 = 0
 F32 = 1
+ZMPO_ERR_ARG = -1
+ZMPO_ERR_HIP = -2
 
 TASK_DTYPE = np.dtype([('c_off', '<i8'), ('tile_base', '<i8'), ('M', '<i4'), ('N', '<i4'),
                        ('c_mlo', '<i4'), ('c_smhi', '<i4'), ('c_smlo', '<i4'), ('c_sn', '<i4'),
@@ -43,7 +45,8 @@ TILE = 64
 
 EXPORTS = ['zmpo_last_error', 'zmpo_abi_version', 'zmpo_struct_sizes', 'zmpo_symAllowed',
            'zmpo_symAllowedInt', 'zmpo_blockgemm', 'zmpo_wavegemm', 'zmpo_shapedgemm', 'zmpo_streamgemm', 'zmpo_chaingemm', 'zmpo_chaingemm16', 'zmpo_skinnygemm', 'zmpo_dotgemm', 'zmpo_dots', 'zmpo_dots_workspace',
-           'zmpo_lincomb', 'zmpo_precond', 'zmpo_gather', 'zmpo_scatter', 'zmpo_convert']
+           'zmpo_lincomb', 'zmpo_precond', 'zmpo_gather', 'zmpo_scatter', 'zmpo_convert',
+           'zmpo_scratch_bytes', 'zmpo_streamgemm_static']
 
 
 class ZmpoError(RuntimeError):
@@ -82,6 +85,9 @@ def lib():
     L.zmpo_chaingemm16.argtypes = L.zmpo_chaingemm.argtypes
     L.zmpo_streamgemm.argtypes = [i32, i32, i32, vp, i64, i32, vp, vp, i64, vp, vp, vp, vp, vp,
                                   i32, vp]
+    L.zmpo_streamgemm_static.argtypes = [i32, i32, i32, vp, i64, i32, vp, vp, i64, vp, vp, vp,
+                                         vp, i64, i32, vp]
+    L.zmpo_scratch_bytes.restype = i64
     L.zmpo_dots.argtypes = [i32, vp, i64, i32, vp, i64, vp, vp, vp]
     L.zmpo_dots_workspace.argtypes = [i32]
     L.zmpo_dots_workspace.restype = i32

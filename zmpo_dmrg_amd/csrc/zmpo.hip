@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/zmpo.h"
@@ -1650,6 +1651,41 @@ void sym_allowed_impl(int rank, int nqnum, const int* ndims, const int* status, 
 
 }  // namespace
 
+// ---------------------------------------------------- allocation-free launch
+// Split-K workspace of zmpo_streamgemm_static: a file-scope device array,
+// one copy per GPU, held until the process ends.  Host code never uses the
+// array's name as an address: its device address is looked up once per
+// device (hipGetSymbolAddress, that device current) and kept in
+// g_scratch_ptr.  No slot needs to start at zero (every slot a launch uses
+// is written in full by its unit before the reduction reads it), so nothing
+// is cleared.  Calls that overlap on one GPU share the array and race.
+static __device__ __align__(256) unsigned char scratch[ZMPO_SCRATCH_BYTES];
+
+namespace {
+constexpr int SCRATCH_MAXDEV = 64;
+void* g_scratch_ptr[SCRATCH_MAXDEV] = {nullptr};
+std::mutex g_scratch_mu;
+
+int scratch_ptr(void** out) {
+  int dev = 0;
+  int rc = check_hip(hipGetDevice(&dev), "streamgemm_static: device");
+  if (rc) return rc;
+  if (dev < 0 || dev >= SCRATCH_MAXDEV)
+    return set_error(ZMPO_ERR_ARG, "streamgemm_static: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                   "streamgemm_static: scratch address");
+    if (rc) return rc;
+    if (!p) return set_error(ZMPO_ERR_HIP, "streamgemm_static: scratch address is null");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return ZMPO_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char* zmpo_last_error(void) { return g_last_error.c_str(); }
@@ -1844,6 +1880,29 @@ int zmpo_streamgemm(int dtype, int shape, int kt, const zmpo_sunit* units, int64
   if (dtype == ZMPO_F32)
     return dispatch_stream<float>(shape, kt, units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
   return set_error(ZMPO_ERR_ARG, "streamgemm: unsupported dtype");
+}
+
+int64_t zmpo_scratch_bytes(void) { return (int64_t)ZMPO_SCRATCH_BYTES; }
+
+int zmpo_streamgemm_static(int dtype, int shape, int kt, const zmpo_sunit* units,
+                           int64_t nunits, int upw, const zmpo_task* tasks,
+                           const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
+                           const void* A, const void* B, void* C, int64_t work_elems, int beta,
+                           void* stream) {
+  const int64_t esz = dtype == ZMPO_F64 ? 8 : dtype == ZMPO_F32 ? 4 : 0;
+  if (!esz) return set_error(ZMPO_ERR_ARG, "streamgemm_static: unsupported dtype");
+  if (work_elems < 0 || (ngroups > 0 && work_elems == 0))
+    return set_error(ZMPO_ERR_ARG, "streamgemm_static: work_elems must cover the split-K slots");
+  if (work_elems > (int64_t)ZMPO_SCRATCH_BYTES / esz)
+    return set_error(ZMPO_ERR_ARG,
+                     "streamgemm_static: split-K workspace exceeds the 64 MiB static scratch");
+  void* work = nullptr;
+  if (ngroups > 0) {
+    int rc = scratch_ptr(&work);
+    if (rc) return rc;
+  }
+  return zmpo_streamgemm(dtype, shape, kt, units, nunits, upw, tasks, groups, ngroups, pairs, A,
+                         B, C, work, beta, stream);
 }
 
 int zmpo_dots(int dtype, const void* X, int64_t ldx, int nx, const void* y, int64_t n,
