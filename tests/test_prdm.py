@@ -14,7 +14,7 @@ from zmpo_dmrg_amd.integrals import gen_spatial, spin_orbital
 def _state(K=4, D=16, seed=3):
     h, eri = gen_spatial(K, seed=seed)
     i1, i2 = spin_orbital(h, eri)
-    d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=D, tol=1e-10)
+    d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=D, tol=1e-10, keep_envs=True)
     pdmrg.run(d, 1)
     pdmrg.sweep(d, 'R')     # every bond has both environments
     return d

@@ -59,7 +59,8 @@ class MPODMRG:
     """The subset of the reference's mpo_dmrg object used on this path."""
 
     def __init__(self, int1e, int2e, nelec, sz=0.0, Dmax=100, tol=1e-8, crit_vec=1e-8,
-                 occun=None, dtype=0, mpo=None, budget=kernelQt.DEFAULT_BUDGET):
+                 occun=None, dtype=0, mpo=None, budget=kernelQt.DEFAULT_BUDGET,
+                 keep_envs=False):
         self.fhop = mpo if mpo is not None else MPOStore(int1e, int2e)
         self.nsite = self.fhop.nsite
         self.sbas = self.fhop.sbas
@@ -80,6 +81,10 @@ class MPODMRG:
         self.sigma_flops = 0
         self.niters = []
         self.timing = {'solve': 0.0, 'renorm': 0.0, 'decim': 0.0}
+        # keep_envs=False: a half-sweep releases the environments it will
+        # not read again (the next half-sweep rebuilds them before use), so
+        # about one environment per bond is resident instead of two
+        self.keep_envs = keep_envs
         K = self.nsite
         occun = default_occupation(K, nelec, sz) if occun is None else occun
         self.qnuml, self.qnumr, s0 = mps0occun(K, occun)
@@ -126,15 +131,43 @@ def sweep_updateBoundary(dmrg, qred, srotR, status):
         dmrg.qnumr[0] = qsym
 
 
+def release_dead_envs(dmrg, isite, status, after):
+    """Drop the environments a half-sweep will not read again.
+
+    'R' (right to left): step isite reads envL[isite-1] and envR[isite+2]
+    and writes envR[isite+1]; envL[j >= isite] are not read again before
+    the next 'L' half-sweep rebuilds them, nor is envL[isite-1] after the
+    step.  'L' mirrors this.  Boundaries (envL[-1], envR[K]) stay."""
+    K = dmrg.nsite
+    if status == 'R':
+        env, first = dmrg.envL, (isite - 1 if after else isite)
+        dead = [j for j in env if j >= max(first, 0)]
+        if not after:
+            dead_r = [isite + 1] if isite + 1 < K and (isite + 1) in dmrg.envR else []
+            for j in dead_r:
+                del dmrg.envR[j]
+    else:
+        env, last = dmrg.envR, (isite + 2 if after else isite + 1)
+        dead = [j for j in env if j <= min(last, K - 1)]
+        if not after and isite in dmrg.envL and isite >= 0:
+            del dmrg.envL[isite]
+    for j in dead:
+        del env[j]
+
+
 def sweep(dmrg, status, max_steps=None):
-    """One half sweep (opt.py:31-95); status 'R' runs right-to-left."""
+    """One half sweep (opt.py:31-95); status 'R' runs right-to-left.  With
+    max_steps the half-sweep stops early and keeps every environment."""
     K = dmrg.nsite
     actlst = list(range(K - 2, -1, -1)) if status == 'R' else list(range(0, K - 1))
     elst, dlst, nmvp = [], [], 0
     qred = srotR = None
+    release = not getattr(dmrg, 'keep_envs', True) and max_steps is None
     for n, isite in enumerate(actlst):
         if max_steps is not None and n >= max_steps:
             break
+        if release:
+            release_dead_envs(dmrg, isite, status, after=False)
         flst = [[dmrg.envL[isite - 1], dmrg.envR[isite + 2]]]
         t0 = time.time()
         nm, eigs, sigs, dwts, qred, site, srotR = ci_solver(dmrg, isite, 2, actlst, flst, status)
@@ -149,6 +182,9 @@ def sweep(dmrg, status, max_steps=None):
             dmrg.envL[isite] = flstN[0][0]
         else:
             dmrg.envR[isite + 1] = flstN[0][0]
+        del flst, flstN
+        if release:
+            release_dead_envs(dmrg, isite, status, after=True)
         t2 = time.time()
         dmrg.timing['solve'] += t1 - t0
         dmrg.timing['renorm'] += t2 - t1
@@ -159,18 +195,29 @@ def sweep(dmrg, status, max_steps=None):
     return nmvp, indx, float(eavs[indx]), elst, dlst
 
 
-def run(dmrg, nsweeps, tol=None):
-    """Fixed2 schedule: sweeps 'R' then 'L'; energy = min of both half-sweep minima
-    (class.py:385-430).  Returns the per-sweep energies."""
+def run(dmrg, nsweeps, tol=None, schedule=None, log=None):
+    """Sweeps 'R' then 'L'; energy = min of both half-sweep minima
+    (class.py:385-430).  Without a schedule: Fixed2 at dmrg.Dmax, stopping
+    when the energy changes by less than tol; with a schedule.Schedule
+    (Fixed2 or Normal2) its parameters are set before every sweep and its
+    convergence test is used (schedule.py:37-93).  log(isweep, eav, dt) is
+    called after every sweep.  Returns the per-sweep energies."""
     tol = dmrg.crit_e if tol is None else tol
     eold = 1e3
     out = []
-    for _ in range(nsweeps):
+    for isweep in range(nsweeps):
+        t0 = time.time()
+        if schedule is not None:
+            schedule.apply(dmrg, isweep)
         r1 = sweep(dmrg, 'R')
         r2 = sweep(dmrg, 'L')
         eav = min(r1[2], r2[2])
         out.append(eav)
-        if abs(eav - eold) < tol:
+        if log is not None:
+            log(isweep, eav, time.time() - t0)
+        conv = schedule.converged(isweep, eav - eold) if schedule is not None \
+            else abs(eav - eold) < tol
+        if conv:
             break
         eold = eav
     return out
