@@ -62,11 +62,17 @@ def run_instance(spec, nsweeps, dtype, budget_gb=6.0):
         energies.append(min(r1[2], r2[2]))
         print('[batch]   instance %d sweep %d L E=%.8f' % (i, isw, r2[2]), flush=True)
     nsigma = d.nsigma
+    # <Psi|H|Psi> of the final MPS (left environment sweep) beside the last
+    # eigenvalue of the final 'L' half-sweep: equal for a consistent MPS
+    from .grad import expectation
+    e_mps = expectation(d.fhop, d.lmps) if nsweeps else None
+    e_last = float(r2[3][-1][0]) if nsweeps else None
     # hand the cached blocks back: workers share one device's memory
     del d
     dv.torch().cuda.empty_cache()
     return {'index': i, 'K': K, 'D': D, 'seed': iseed, 'energies': energies,
-            'sweep_s': times, 'setup_s': setup, 'nsigma': nsigma, 'pid': os.getpid()}
+            'sweep_s': times, 'setup_s': setup, 'nsigma': nsigma, 'pid': os.getpid(),
+            'e_mps': e_mps, 'e_last': e_last}
 
 
 def _worker(args):

@@ -33,6 +33,7 @@ torch.autograd.Function.
 """
 import numpy as np
 
+from . import _lib
 from . import device as dv
 from . import kernelQt as kq
 from . import mpo as pmpo
@@ -195,16 +196,22 @@ class WGradProgram:
             self.stages.append(progs)
             self.inter = [max(a, b) for a, b in zip(self.inter, offs)]
 
-    def apply(self, site_values, envL, envR):
-        """Device buffer with z of op p at out_off[p]."""
-        x = dv.empty(self.inter[0], 0)
-        y = dv.empty(self.inter[1], 0)
-        out = dv.zeros(self.out_size, 0)
-        EL, ER = envL.buffer(0), envR.buffer(0)
+    def apply(self, site_values, envL, envR, dtype=0):
+        """Device buffer (FP64) with z of op p at out_off[p]; dtype 1 runs the
+        three launches with FP32 operands (site_values then FP32 too)."""
+        x = dv.empty(self.inter[0], dtype)
+        y = dv.empty(self.inter[1], dtype)
+        out = dv.zeros(self.out_size, dtype)
+        EL, ER = envL.buffer(dtype), envR.buffer(dtype)
         for st in self.stages:
-            qq.run_program(st[0], EL, site_values, x, 0)
-            qq.run_program(st[1], ER, site_values, y, 0)
-            qq.run_program(st[2], x, y, out, 0)
+            qq.run_program(st[0], EL, site_values, x, dtype)
+            qq.run_program(st[1], ER, site_values, y, dtype)
+            qq.run_program(st[2], x, y, out, dtype)
+        if dtype != 0:
+            o64 = dv.empty(self.out_size, 0)
+            _lib.check(_lib.lib().zmpo_convert(0, dv.ptr(o64), dtype, dv.ptr(out),
+                                               self.out_size, dv.stream()))
+            out = o64
         return out
 
     def dense(self, mpo, values):
@@ -220,32 +227,36 @@ class WGradProgram:
         return out
 
 
-def right_environments(mpo, sites, qkey):
+def right_environments(mpo, sites, qkey, dtype=0):
     """R_p[j] for j = K .. 1 over the left-canonical sites relabelled as right
-    sites (the cotangents of the left environments)."""
+    sites (the cotangents of the left environments); dtype 1: FP32 GEMMs,
+    environments kept in FP64."""
     K = mpo.nsite
     envs = {K: kq.boundary_store(mpo.nops)}
     for j in range(K - 1, 0, -1):
-        envs[j], _ = kq.renorm(mpo, j, as_right_site(sites[j], qkey), envs[j + 1], 'R')
+        envs[j], _ = kq.renorm(mpo, j, as_right_site(sites[j], qkey), envs[j + 1], 'R', dtype)
     return envs
 
 
-def w_gradients(mpo, sites, qkey, hpwts=None):
+def w_gradients(mpo, sites, qkey, hpwts=None, dtype=0):
     """Yield (j, [dE/dW_p[j] dense sorted, for each op]) for j = 0 .. K-1.
 
     Also checks the invariant  sum_p <dE/dW_p[j], W_p[j]> = E  at every site
     (E is linear in each W[j]); the per-site values are collected in
-    ``w_gradients.last_energies``."""
+    ``w_gradients.last_energies``.  dtype 1: every GEMM of the pass (the
+    environment recomputation and the three dE/dW launches) with FP32
+    operands; environments and results are held in FP64."""
     K = mpo.nsite
     hpwts = mpo.hpwts if hpwts is None else hpwts
-    envR = right_environments(mpo, sites, qkey)
+    envR = right_environments(mpo, sites, qkey, dtype)
     envL = kq.boundary_store(mpo.nops)
     energies = []
     for j in range(K):
         s = sites[j]
-        sv = dv.upload(s.value, 0) if isinstance(s.value, np.ndarray) else s.value
+        sv = dv.upload(s.value, dtype) if isinstance(s.value, np.ndarray) else \
+            (s.value if dtype == 0 else s.value.float())
         prog = WGradProgram(mpo, j, s, envL, envR[j + 1], qkey, hpwts)
-        vals = dv.download(prog.apply(sv, envL, envR[j + 1]))
+        vals = dv.download(prog.apply(sv, envL, envR[j + 1], dtype))
         g = prog.dense(mpo, vals)
         e = 0.0
         for p in range(mpo.nops):
@@ -254,23 +265,24 @@ def w_gradients(mpo, sites, qkey, hpwts=None):
         energies.append(e)
         yield j, g
         if j < K - 1:
-            envL, _ = kq.renorm(mpo, j, s, envL, 'L')
+            envL, _ = kq.renorm(mpo, j, s, envL, 'L', dtype)
     w_gradients.last_energies = energies
 
 
 w_gradients.last_energies = []
 
 
-def integral_gradients(mpo, sites, qkey, h, eri):
+def integral_gradients(mpo, sites, qkey, h, eri, dtype=0):
     """(E, dE/dh[K,K], dE/deri[K,K,K,K]) of <Psi|H(h, eri)|Psi> for the MPS
-    ``sites`` (left-canonical, normalised), raw (unsymmetrised) gradients."""
+    ``sites`` (left-canonical, normalised), raw (unsymmetrised) gradients;
+    dtype 1 runs the device pass with FP32 GEMM operands."""
     torch = dv.torch()
     # also called from a torch.autograd.Function backward, where grad mode is off
     with torch.enable_grad():
-        return _integral_gradients(torch, mpo, sites, qkey, h, eri)
+        return _integral_gradients(torch, mpo, sites, qkey, h, eri, dtype)
 
 
-def _integral_gradients(torch, mpo, sites, qkey, h, eri):
+def _integral_gradients(torch, mpo, sites, qkey, h, eri, dtype=0):
     ht = torch.as_tensor(np.asarray(h, dtype=np.float64)).clone().requires_grad_(True)
     et = torch.as_tensor(np.asarray(eri, dtype=np.float64)).clone().requires_grad_(True)
     int1e, int2e = spin_orbital_t(ht, et)
@@ -278,7 +290,7 @@ def _integral_gradients(torch, mpo, sites, qkey, h, eri):
     i2 = int2e.detach().clone().requires_grad_(True)
     nsite = 2 * mpo.nsite
     ops = list(range(mpo.nops))
-    for j, g in w_gradients(mpo, sites, qkey):
+    for j, g in w_gradients(mpo, sites, qkey, dtype=dtype):
         w = site_dense_t(ops, nsite, j, i1, i2)
         G = np.zeros(tuple(w.shape))
         for p in ops:
@@ -299,13 +311,13 @@ def sym8(g):
     return sum(g.transpose(pm) for pm in _PERM8) / 8.0
 
 
-def rdms(mpo, sites, qkey, h, eri):
+def rdms(mpo, sites, qkey, h, eri, dtype=0):
     """(E, gamma[K,K], Gamma[K,K,K,K]): the spin-summed 1-RDM gamma_pr = <E_pr>
     and the 8-fold symmetrised 2-RDM, so that E = sum h gamma + 1/2 sum (pr|qs)
     Gamma for 8-fold symmetric eri.  Along a perturbation of eri that keeps
     the symmetry, dE/d(pr|qs) summed over the orbit of (pr|qs) is
     1/2 sum_orbit Gamma = |orbit| sym8(dE/deri)."""
-    E, gh, ge = integral_gradients(mpo, sites, qkey, h, eri)
+    E, gh, ge = integral_gradients(mpo, sites, qkey, h, eri, dtype)
     return E, gh, 2.0 * sym8(ge)
 
 

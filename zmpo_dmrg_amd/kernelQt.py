@@ -72,6 +72,14 @@ def case(porb, isite, ncsite=1):
     return 6 + porb % 2
 
 
+def op_case(mpo, p, isite):
+    """Class of operator p at one site for the one-site contractions
+    (renormalisation): ops of one class share their plans.  Stores of
+    other operators than H (mpo.op_class) define their own classes."""
+    f = getattr(mpo, 'op_class', None)
+    return f(p, isite) if f is not None else case(mpo.opers[p], isite)
+
+
 # ------------------------------------------------------------------ stores
 class EnvStore:
     """Environments of one bond for every op: ``meta[p]`` (QTensor, ENV_PERM)
@@ -592,7 +600,7 @@ class RenormProgram:
         self.site_idx = site_idx
         self.status = status
         plans = {}
-        case_of = [case(mpo.opers[p], site_idx) for p in range(nops)]
+        case_of = [op_case(mpo, p, site_idx) for p in range(nops)]
         for p in range(nops):
             c = case_of[p]
             if c in plans:
