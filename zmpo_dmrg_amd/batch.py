@@ -97,13 +97,16 @@ def summarize(results, wall, nsweeps):
 THREAD_VARS = ('OMP_NUM_THREADS', 'OPENBLAS_NUM_THREADS', 'MKL_NUM_THREADS')
 
 
-def run_batch(specs, workers, nsweeps, dtype, cores=16, budget_gb=6.0):
+def run_batch(specs, workers, nsweeps, dtype, cores=16, budget_gb=6.0, spawn=False):
     """Run the instances on ``workers`` processes (spawned: each initialises
     its own HIP context) with cores // workers host threads each, so the
     workers' BLAS and torch thread pools do not oversubscribe the ``cores``
-    host cores; returns (results sorted by index, summary)."""
+    host cores; returns (results sorted by index, summary).  workers = 1
+    runs in this process unless spawn is set.  Host BLAS results (the
+    decimation's eigh and products) can depend on the thread count, so runs
+    compared bit for bit need the same threads per worker."""
     t = time.time()
-    if workers <= 1:
+    if workers <= 1 and not spawn:
         results = [_worker((s, nsweeps, dtype, budget_gb)) for s in specs]
     else:
         saved = {k: os.environ.get(k) for k in THREAD_VARS}

@@ -98,6 +98,21 @@ class MPODMRG:
         self.envL, _ = kernelQt.genHops(self.fhop, self.lmps, 'L')
 
 
+def restart_from_left_mps(dmrg, sites, qnuml):
+    """Continue from a left-canonical MPS (the state after an 'L' half-sweep;
+    the last site holds the normalised coefficients): its sites and bond
+    QNs replace the product state, every left environment is rebuilt
+    (initOps, mpo_dmrg_util.py:193-214) and no guess is carried over."""
+    K = dmrg.nsite
+    for i in range(K):
+        dmrg.lmps[i] = sites[i]
+    for i in range(K + 1):
+        dmrg.qnuml[i] = np.asarray(qnuml[i])
+    dmrg.envL, _ = kernelQt.genHops(dmrg.fhop, dmrg.lmps, 'L')
+    dmrg.envR = {K: kernelQt.boundary_store(dmrg.nops)}
+    dmrg.psi0 = None
+
+
 def sweep_updateSite(dmrg, isite, qred, site, status):
     """opt.py:313-344 (ncsite=2)."""
     if status == 'L':
@@ -188,6 +203,9 @@ def sweep(dmrg, status, max_steps=None):
         t2 = time.time()
         dmrg.timing['solve'] += t1 - t0
         dmrg.timing['renorm'] += t2 - t1
+        log = getattr(dmrg, 'step_log', None)
+        if log is not None:
+            log(status, isite, float(eigs[0]), t2 - t0, dmrg.nsigma)
     if max_steps is None:
         sweep_updateBoundary(dmrg, qred, srotR, status)
     eavs = np.array([e[0] for e in elst])
