@@ -86,20 +86,23 @@ def build_workload(K, D, seed, dtype, budget, fused=True, extra=None):
 
 
 def time_stages(prog, mpo, L, R, psi, sig, dtype, reps=3):
+    """dtype 2: BF16 operands, FP32 sigma (the *R launch runs as ZMPO_BF16_F32)."""
     """Per-launch average time of each sigma kernel, with HIP events recorded
     on the stream the kernels are launched on, plus its algorithmic FLOPs and
     bytes.  Launches (per op chunk): L*psi, then either the chained
     W[i]*W[i+1] launch or its two separate stages, then *R into sigma."""
     from zmpo_dmrg_amd import device as dv
+    from zmpo_dmrg_amd.kernelQt import out_dtype
     from zmpo_dmrg_amd.qtensor import run_chain, run_program
     torch = dv.torch()
     fused = prog.fused
+    odt = out_dtype(dtype)
     t1, t2, t3 = prog.buffers(dtype, mid=not fused)
     Lb, Rb = L.buffer(dtype), R.buffer(dtype)
     Wi, Wj = mpo.buffer(prog.isite, dtype), mpo.buffer(prog.isite + 1, dtype)
     if fused:
         Wpi, Wpj = mpo.padded_buffer(prog.isite, dtype), mpo.padded_buffer(prog.isite + 1, dtype)
-    elsize = 8 if dtype == 0 else 4
+    elsize = {0: 8, 1: 4, 2: 2}[dtype]
     if fused:
         names = ['L*psi', 'W[i]*W[i+1] chained', '*R']
     else:
@@ -112,12 +115,12 @@ def time_stages(prog, mpo, L, R, psi, sig, dtype, reps=3):
             if fused:
                 ops = [(lambda: run_program(st[0], Lb, psi, t1, dtype, beta=0), st[0]),
                        (lambda: run_chain(ch, t1, Wpi, Wpj, t3, dtype), ch),
-                       (lambda: run_program(st[3], t3, Rb, sig, dtype, beta=0), st[3])]
+                       (lambda: run_program(st[3], t3, Rb, sig, odt, beta=0), st[3])]
             else:
                 ops = [(lambda: run_program(st[0], Lb, psi, t1, dtype, beta=0), st[0]),
                        (lambda: run_program(st[1], t1, Wi, t2, dtype, beta=0), st[1]),
                        (lambda: run_program(st[2], t2, Wj, t3, dtype, beta=0), st[2]),
-                       (lambda: run_program(st[3], t3, Rb, sig, dtype, beta=0), st[3])]
+                       (lambda: run_program(st[3], t3, Rb, sig, odt, beta=0), st[3])]
             for k, (fn, pg) in enumerate(ops):
                 ev0 = torch.cuda.Event(enable_timing=True)
                 ev1 = torch.cuda.Event(enable_timing=True)
@@ -302,6 +305,22 @@ def main():
         prec = {'f32_ms_per_step': ms32, 'f32_gflops': prog.flops / ms32 / 1e6,
                 'f32_speedup_vs_f64': ms_step / ms32, 'f32_rel_err': err}
         log('FP32 sigma: %.3f ms/step (%.2fx FP64), relative error %.2e' % (ms32, ms_step / ms32, err))
+        # BF16 operands, FP32 accumulation (BF16 MFMA), FP32 sigma
+        psi16 = dv.upload(psi_h, 2)
+        prog.apply(mpo, L, R, psi16, sig32, 2)
+        dv.sync()
+        t = time.time()
+        for _ in range(args.steps):
+            prog.apply(mpo, L, R, psi16, sig32, 2)
+        dv.sync()
+        ms16 = 1e3 * (time.time() - t) / args.steps
+        err16 = float(np.linalg.norm(dv.download(sig32).astype(np.float64) - s64) / np.linalg.norm(s64))
+        names16, st_ms16, _, _ = time_stages(prog, mpo, L, R, psi16, sig32, 2)
+        prec.update({'bf16_ms_per_step': ms16, 'bf16_gflops': prog.flops / ms16 / 1e6,
+                     'bf16_speedup_vs_f64': ms_step / ms16, 'bf16_rel_err': err16,
+                     'bf16_stage_ms': [round(x, 4) for x in st_ms16]})
+        log('BF16 sigma: %.3f ms/step (%.2fx FP64), relative error %.2e, stages %s' % (
+            ms16, ms_step / ms16, err16, st_ms16))
     cpu = None
     if not args.no_cpu_baseline:
         try:

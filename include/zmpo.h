@@ -61,6 +61,13 @@ extern "C" {
 
 #define ZMPO_F64 0
 #define ZMPO_F32 1
+/* BF16 operands (round to nearest even), FP32 accumulation on the BF16
+ * MFMA; ZMPO_BF16 writes BF16 outputs, ZMPO_BF16_F32 FP32 outputs and
+ * split-K workspace.  Accepted by zmpo_shapedgemm, zmpo_streamgemm(_static)
+ * (both codes), zmpo_chaingemm(16) (ZMPO_BF16) and zmpo_convert (ZMPO_BF16
+ * to and from FP64 / FP32; FP64 rounds through FP32). */
+#define ZMPO_BF16 2
+#define ZMPO_BF16_F32 3
 
 /* One output block of a grouped block GEMM.  Output element (m, n) of the
  * block lives at c_off + (m / c_mlo) * c_smhi + (m % c_mlo) * c_smlo

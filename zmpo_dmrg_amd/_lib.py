@@ -16,6 +16,8 @@ LIB_PATH = os.environ.get('ZMPO_LIB') or os.path.join(_HERE, 'libzmpo.so')
 
 F64 = 0
 F32 = 1
+ZMPO_BF16 = 2        # BF16 operands and outputs, FP32 accumulation
+ZMPO_BF16_F32 = 3    # BF16 operands, FP32 outputs and split-K workspace
 ZMPO_ERR_ARG = -1
 ZMPO_ERR_HIP = -2
 

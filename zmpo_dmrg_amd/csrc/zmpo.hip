@@ -48,14 +48,57 @@ __device__ __forceinline__ int wave_id() {
 }
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
+// BF16 storage (the upper 16 bits of an IEEE binary32).  Values are rounded
+// to nearest, ties to even, when stored; arithmetic is FP32.
+struct bf16s {
+  unsigned short x;
+};
+typedef short sh4 __attribute__((ext_vector_type(4)));
+
+// Loads, scaling and stores of one storage type T with arithmetic in M.
+template <typename T>
+struct Num {
+  typedef T M;
+  __device__ static inline T zero() { return (T)0; }
+  __device__ static inline M get(T v) { return v; }
+  __device__ static inline T put(M v) { return v; }
+};
+template <>
+struct Num<bf16s> {
+  typedef float M;
+  __device__ static inline bf16s zero() { return bf16s{0}; }
+  __device__ static inline float get(bf16s v) { return __uint_as_float((unsigned)v.x << 16); }
+  __device__ static inline bf16s put(float f) {
+    // round to nearest even on the 16 dropped bits (inputs are finite)
+    const unsigned u = __float_as_uint(f);
+    return bf16s{(unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16)};
+  }
+};
+// v * scale in the arithmetic type, stored back in T (exact when scale is 1)
+template <typename T>
+__device__ __forceinline__ T scaled(T v, typename Num<T>::M sc) {
+  return Num<T>::put(Num<T>::get(v) * sc);
+}
+// store an accumulated value into an output element (beta: add to it)
+template <typename TC, typename V>
+__device__ __forceinline__ void store_acc(TC* q, V v, int beta) {
+  typedef typename Num<TC>::M M;
+  *q = beta ? Num<TC>::put(Num<TC>::get(*q) + (M)v) : Num<TC>::put((M)v);
+}
+
 template <typename T>
 struct Acc;
 // Result element r of lane l lands at row(l/16, r), column l%16 of the 16x16
 // tile: FP64 interleaves rows across lane groups, FP32 keeps four consecutive
-// rows per lane group.
+// rows per lane group.  KG = k-steps of 4 consumed by one MFMA: operand
+// element (row lr, k-step s) of lane group lk holds k = 4 s + lk, so one
+// BF16 16x16x16 MFMA takes four k-steps, lane group lk supplying k = lk,
+// lk + 4, lk + 8, lk + 12 of both operands (the same relabelling of k on
+// both sides leaves the sum unchanged).
 template <>
 struct Acc<double> {
   typedef dbl4 type;
+  static constexpr int KG = 1;
   __device__ static inline dbl4 mfma(double a, double b, dbl4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -64,8 +107,21 @@ struct Acc<double> {
 template <>
 struct Acc<float> {
   typedef flt4 type;
+  static constexpr int KG = 1;
   __device__ static inline flt4 mfma(float a, float b, flt4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lk, int r) { return 4 * lk + r; }
+};
+template <>
+struct Acc<bf16s> {
+  typedef flt4 type;
+  static constexpr int KG = 4;
+  __device__ static inline flt4 mfma4(bf16s a0, bf16s a1, bf16s a2, bf16s a3, bf16s b0,
+                                      bf16s b1, bf16s b2, bf16s b3, flt4 c) {
+    const sh4 a = {(short)a0.x, (short)a1.x, (short)a2.x, (short)a3.x};
+    const sh4 b = {(short)b0.x, (short)b1.x, (short)b2.x, (short)b3.x};
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
   }
   __device__ static inline int row(int lk, int r) { return 4 * lk + r; }
 };
@@ -372,15 +428,16 @@ int launch_wave(const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
 // threads read consecutive addresses.
 template <typename T, int ROWS, int KT>
 struct TileLoader {
+  typedef typename Num<T>::M M;
   static constexpr int NE = ROWS * KT / 256;
   int64_t roff[NE];  // row offset (relative to base), -1 = padding row
   int rr[NE], kk[NE];
   const T* base;
   int64_t ks;
   int K;
-  T scale;
+  M scale;
   __device__ __forceinline__ void setup(int tid, const T* b, int64_t off, int K_, int64_t sk, int mlo,
-                        int64_t shi, int64_t slo, int r0, int rmax, T sc) {
+                        int64_t shi, int64_t slo, int r0, int rmax, M sc) {
     base = b + off;
     ks = sk;
     K = K_;
@@ -407,7 +464,8 @@ struct TileLoader {
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
       const int k = k0 + kk[i];
-      dst[kk[i]][rr[i]] = (roff[i] >= 0 && k < K) ? base[roff[i] + (int64_t)k * ks] * scale : (T)0;
+      dst[kk[i]][rr[i]] = (roff[i] >= 0 && k < K) ? scaled(base[roff[i] + (int64_t)k * ks], scale)
+                                                  : Num<T>::zero();
     }
   }
 };
@@ -415,24 +473,26 @@ struct TileLoader {
 // One workgroup = one unit; its pairs' k-tiles form one flat, double-buffered
 // pipeline (the next tile -- possibly of the next pair -- is loaded while the
 // current one feeds the MFMAs), so a unit pays the memory latency about once.
-template <typename T, int TMv, int TNv>
+template <typename T, typename TC, int TMv, int TNv>
 __global__ __launch_bounds__(256) void shaped_gemm_kernel(
     const zmpo_task* __restrict__ tasks, const zmpo_unit* __restrict__ units, int64_t nunits,
     const zmpo_pair* __restrict__ pairs, const T* __restrict__ A, const T* __restrict__ B,
-    T* __restrict__ C, T* __restrict__ work, int beta) {
+    TC* __restrict__ C, TC* __restrict__ work, int beta) {
   typedef typename Acc<T>::type accv;
+  typedef typename Num<T>::M M;
   constexpr int KT = 16;
+  constexpr int KG = Acc<T>::KG;
   constexpr int BM = TMv / 16, BN = TNv / 16;
   constexpr int NBLK = BM * BN;
   constexpr int PER = (NBLK + 3) / 4;
   // the output staging tile reuses the operand buffers (the k loop ends with
   // a barrier): less LDS per workgroup, more workgroups per CU
   constexpr int NA = 2 * KT * (TMv + 1), NB = 2 * KT * (TNv + 1);
-  constexpr int NS = NA + NB > TMv * TNv ? NA + NB : TMv * TNv;
-  __shared__ T smem[NS];
+  constexpr int NOPB = (NA + NB) * (int)sizeof(T), NSTB = TMv * TNv * (int)sizeof(TC);
+  __shared__ __align__(16) unsigned char smem[NOPB > NSTB ? NOPB : NSTB];
   T(*As)[KT][TMv + 1] = reinterpret_cast<T(*)[KT][TMv + 1]>(smem);
-  T(*Bs)[KT][TNv + 1] = reinterpret_cast<T(*)[KT][TNv + 1]>(smem + NA);
-  T* stage = smem;
+  T(*Bs)[KT][TNv + 1] = reinterpret_cast<T(*)[KT][TNv + 1]>(smem + NA * sizeof(T));
+  TC* stage = reinterpret_cast<TC*>(smem);
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = wave_id();
   const int lr = lane & 15, lk = lane >> 4;
@@ -451,8 +511,8 @@ __global__ __launch_bounds__(256) void shaped_gemm_kernel(
   {                                                                                              \
     const zmpo_pair pr = pairs[(P)];                                                             \
     la.setup(tid, A, pr.a_off, pr.K, pr.a_sk, pr.a_mlo, pr.a_smhi, pr.a_smlo, un.m0, tk.M,      \
-             (T)pr.scale);                                                                       \
-    lb.setup(tid, B, pr.b_off, pr.K, pr.b_sk, pr.b_nlo, pr.b_snhi, pr.b_snlo, un.n0, tk.N, (T)1); \
+             (M)pr.scale);                                                                       \
+    lb.setup(tid, B, pr.b_off, pr.K, pr.b_sk, pr.b_nlo, pr.b_snhi, pr.b_snlo, un.n0, tk.N, (M)1); \
     lK = pr.K;                                                                                   \
   }
   int ks_cur = 0;  // k-steps held by the buffer being computed (0 = done)
@@ -480,15 +540,23 @@ __global__ __launch_bounds__(256) void shaped_gemm_kernel(
       ++lkt;
     }
 #pragma unroll
-    for (int kk = 0; kk < KT; kk += 4) {
+    for (int kk = 0; kk < KT; kk += 4 * KG) {
       if (kk >= ks_cur) break;  // uniform
 #pragma unroll
       for (int i = 0; i < PER; ++i) {
         const int blkid = wv + 4 * i;
         if (blkid < NBLK) {
           const int bm = blkid % BM, bn = blkid / BM;
-          acc[i] = Acc<T>::mfma(As[buf][kk + lk][bm * 16 + lr], Bs[buf][kk + lk][bn * 16 + lr],
-                                acc[i]);
+          const int am = bm * 16 + lr, bnn = bn * 16 + lr;
+          if constexpr (KG == 1) {
+            acc[i] = Acc<T>::mfma(As[buf][kk + lk][am], Bs[buf][kk + lk][bnn], acc[i]);
+          } else {
+            // k rows past ks_cur hold zeros in both tiles
+            acc[i] = Acc<T>::mfma4(As[buf][kk + lk][am], As[buf][kk + 4 + lk][am],
+                                   As[buf][kk + 8 + lk][am], As[buf][kk + 12 + lk][am],
+                                   Bs[buf][kk + lk][bnn], Bs[buf][kk + 4 + lk][bnn],
+                                   Bs[buf][kk + 8 + lk][bnn], Bs[buf][kk + 12 + lk][bnn], acc[i]);
+          }
         }
       }
     }
@@ -505,12 +573,13 @@ __global__ __launch_bounds__(256) void shaped_gemm_kernel(
       const int bm = blkid % BM, bn = blkid / BM;
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        stage[(bm * 16 + Acc<T>::row(lk, r)) * TNv + bn * 16 + lr] = acc[i][r];
+        stage[(bm * 16 + Acc<T>::row(lk, r)) * TNv + bn * 16 + lr] =
+            Num<TC>::put((typename Num<TC>::M)acc[i][r]);
     }
   }
   __syncthreads();
   if (un.slot >= 0) {
-    T* __restrict__ w = work + (int64_t)un.slot * (TMv * TNv);
+    TC* __restrict__ w = work + (int64_t)un.slot * (TMv * TNv);
     for (int e = tid; e < TMv * TNv; e += 256) w[e] = stage[e];
     return;
   }
@@ -524,8 +593,7 @@ __global__ __launch_bounds__(256) void shaped_gemm_kernel(
     const int64_t ro = one ? (int64_t)mi * tk.c_smlo
                            : (int64_t)(mi / tk.c_mlo) * tk.c_smhi + (int64_t)(mi % tk.c_mlo) * tk.c_smlo;
     const int64_t off = tk.c_off + ro + (int64_t)ni * tk.c_sn;
-    const T v = stage[e];
-    C[off] = beta ? C[off] + v : v;
+    store_acc(C + off, Num<TC>::get(stage[e]), beta);
   }
 }
 
@@ -533,6 +601,7 @@ template <typename T, int TMv, int TNv>
 __global__ __launch_bounds__(256) void shaped_reduce_kernel(
     const zmpo_task* __restrict__ tasks, const zmpo_rgroup* __restrict__ groups, int64_t ngroups,
     const T* __restrict__ work, T* __restrict__ C, int beta) {
+  typedef typename Num<T>::M M;
   const int64_t g = blockIdx.x;
   if (g >= ngroups) return;
   const zmpo_rgroup gr = groups[g];
@@ -542,46 +611,46 @@ __global__ __launch_bounds__(256) void shaped_reduce_kernel(
   for (int e = threadIdx.x; e < TMv * TNv; e += blockDim.x) {
     const int row = e / TNv, col = e % TNv;
     if (row >= mv || col >= nv) continue;
-    T s = 0;
+    M s = 0;
     for (int q = 0; q < gr.nslots; ++q)
-      s += work[(int64_t)(gr.slot_begin + q) * (TMv * TNv) + e];
+      s += Num<T>::get(work[(int64_t)(gr.slot_begin + q) * (TMv * TNv) + e]);
     const int mi = gr.m0 + row, ni = gr.n0 + col;
     const int64_t off = tk.c_off + (int64_t)(mi / tk.c_mlo) * tk.c_smhi +
                         (int64_t)(mi % tk.c_mlo) * tk.c_smlo + (int64_t)ni * tk.c_sn;
-    C[off] = beta ? C[off] + s : s;
+    store_acc(C + off, s, beta);
   }
 }
 
-template <typename T, int TMv, int TNv>
+template <typename T, typename TC, int TMv, int TNv>
 int launch_shaped(const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
                   const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
                   const void* A, const void* B, void* C, void* work, int beta, hipStream_t s) {
   if (nunits > 0) {
-    hipLaunchKernelGGL((shaped_gemm_kernel<T, TMv, TNv>), dim3((unsigned)nunits), dim3(256), 0, s,
-                       tasks, units, nunits, pairs, (const T*)A, (const T*)B, (T*)C, (T*)work,
-                       beta);
+    hipLaunchKernelGGL((shaped_gemm_kernel<T, TC, TMv, TNv>), dim3((unsigned)nunits), dim3(256), 0,
+                       s, tasks, units, nunits, pairs, (const T*)A, (const T*)B, (TC*)C,
+                       (TC*)work, beta);
     int rc = check_hip(hipGetLastError(), "shaped_gemm launch");
     if (rc) return rc;
   }
   if (ngroups > 0) {
-    hipLaunchKernelGGL((shaped_reduce_kernel<T, TMv, TNv>), dim3((unsigned)ngroups), dim3(256), 0,
-                       s, tasks, groups, ngroups, (const T*)work, (T*)C, beta);
+    hipLaunchKernelGGL((shaped_reduce_kernel<TC, TMv, TNv>), dim3((unsigned)ngroups), dim3(256), 0,
+                       s, tasks, groups, ngroups, (const TC*)work, (TC*)C, beta);
     return check_hip(hipGetLastError(), "shaped_reduce launch");
   }
   return ZMPO_OK;
 }
 
-template <typename T>
+template <typename T, typename TC = T>
 int dispatch_shaped(int shape, const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
                     const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
                     const void* A, const void* B, void* C, void* work, int beta, hipStream_t s) {
   switch (shape) {
-    case ZMPO_TILE_64x16: return launch_shaped<T, 64, 16>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
-    case ZMPO_TILE_128x16: return launch_shaped<T, 128, 16>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
-    case ZMPO_TILE_32x32: return launch_shaped<T, 32, 32>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
-    case ZMPO_TILE_64x32: return launch_shaped<T, 64, 32>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
-    case ZMPO_TILE_16x16: return launch_shaped<T, 16, 16>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
-    case ZMPO_TILE_64x64: return launch_shaped<T, 64, 64>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_64x16: return launch_shaped<T, TC, 64, 16>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_128x16: return launch_shaped<T, TC, 128, 16>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_32x32: return launch_shaped<T, TC, 32, 32>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_64x32: return launch_shaped<T, TC, 64, 32>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_16x16: return launch_shaped<T, TC, 16, 16>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_64x64: return launch_shaped<T, TC, 64, 64>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
     default: return set_error(ZMPO_ERR_ARG, "shapedgemm: unknown tile shape");
   }
 }
@@ -601,6 +670,7 @@ constexpr int MAXUPW = 64;
 // threads read consecutive addresses.
 template <typename T, int ROWS, int KT>
 struct StreamLoader {
+  typedef typename Num<T>::M M;
   static constexpr int NE = (ROWS * KT + 255) / 256;
   int roff[NE];  // row offset relative to the block, -1 = padding row
   T v[NE];
@@ -609,12 +679,12 @@ struct StreamLoader {
   int K;
   bool kmaj;   // element layout of the pair set up last
   bool vkmaj;  // element layout of the values held in v (set by fetch)
-  T scale;
+  M scale;
   __device__ __forceinline__ static int kidx(bool km, int e) { return km ? e % KT : e / ROWS; }
   __device__ __forceinline__ static int ridx(bool km, int e) { return km ? e / KT : e % ROWS; }
   __device__ __forceinline__ void setup(int tid, const T* b, int64_t off, int K_, int64_t sk,
                                         int mlo, int64_t shi, int64_t slo, int r0, int rmax,
-                                        T sc) {
+                                        M sc) {
     base = b + off;
     ks = sk;
     K = K_;
@@ -635,7 +705,8 @@ struct StreamLoader {
     for (int i = 0; i < NE; ++i) {
       const int e = tid + 256 * i;
       const int k = k0 + kidx(kmaj, e);
-      v[i] = (roff[i] >= 0 && k < K) ? base[roff[i] + (int64_t)k * ks] * scale : (T)0;
+      v[i] = (roff[i] >= 0 && k < K) ? scaled(base[roff[i] + (int64_t)k * ks], scale)
+                                     : Num<T>::zero();
     }
   }
   __device__ __forceinline__ void store(int tid, T (*dst)[ROWS + 1]) const {
@@ -647,22 +718,26 @@ struct StreamLoader {
   }
 };
 
-template <typename T, int TMv, int TNv, int KT>
+template <typename T, typename TC, int TMv, int TNv, int KT>
 __global__ __launch_bounds__(256) void stream_gemm_kernel(
     const zmpo_sunit* __restrict__ units, int64_t nunits, int upw,
     const zmpo_pair* __restrict__ pairs, const T* __restrict__ A, const T* __restrict__ B,
-    T* __restrict__ C, T* __restrict__ work, int beta) {
+    TC* __restrict__ C, TC* __restrict__ work, int beta) {
   typedef typename Acc<T>::type accv;
+  typedef typename Num<T>::M M;
+  constexpr int KG = Acc<T>::KG;
+  static_assert(KT % (4 * KG) == 0, "k-tile must hold whole MFMA k blocks");
   constexpr int BM = TMv / 16, BN = TNv / 16;
   constexpr int NBLK = BM * BN;
   constexpr int PER = (NBLK + 3) / 4;
-  constexpr int ASZ = KT * (TMv + 1), STG = TMv * TNv;
+  constexpr int ASZ = KT * (TMv + 1) * (int)sizeof(T), STG = TMv * TNv * (int)sizeof(TC);
   constexpr int ABUF = ASZ > STG ? ASZ : STG;
   // the A tile buffer doubles as the output stage of a finished unit
-  __shared__ T Abuf[ABUF];
+  __shared__ __align__(16) unsigned char Abuf_raw[ABUF];
   __shared__ T Bs[KT][TNv + 1];
   __shared__ zmpo_sunit su[MAXUPW];
-  T(*As)[TMv + 1] = reinterpret_cast<T(*)[TMv + 1]>(Abuf);
+  T(*As)[TMv + 1] = reinterpret_cast<T(*)[TMv + 1]>(Abuf_raw);
+  TC* Abuf = reinterpret_cast<TC*>(Abuf_raw);
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = wave_id();
   const int lr = lane & 15, lk = lane >> 4;
@@ -680,9 +755,9 @@ __global__ __launch_bounds__(256) void stream_gemm_kernel(
   StreamLoader<T, TMv, KT> la;
   StreamLoader<T, TNv, KT> lb;
 #pragma unroll
-  for (int i = 0; i < la.NE; ++i) la.v[i] = (T)0;
+  for (int i = 0; i < la.NE; ++i) la.v[i] = Num<T>::zero();
 #pragma unroll
-  for (int i = 0; i < lb.NE; ++i) lb.v[i] = (T)0;
+  for (int i = 0; i < lb.NE; ++i) lb.v[i] = Num<T>::zero();
   la.vkmaj = lb.vkmaj = true;
   accv acc[PER];
 #pragma unroll
@@ -695,9 +770,9 @@ __global__ __launch_bounds__(256) void stream_gemm_kernel(
   {                                                                                           \
     const zmpo_pair pr = pairs[lp];                                                           \
     la.setup(tid, A, pr.a_off, pr.K, pr.a_sk, pr.a_mlo, pr.a_smhi, pr.a_smlo, su[lu].m0,      \
-             su[lu].M, (T)pr.scale);                                                          \
+             su[lu].M, (M)pr.scale);                                                          \
     lb.setup(tid, B, pr.b_off, pr.K, pr.b_sk, pr.b_nlo, pr.b_snhi, pr.b_snlo, su[lu].n0,      \
-             su[lu].N, (T)1);                                                                 \
+             su[lu].N, (M)1);                                                                 \
     lK = pr.K;                                                                                \
   }
 #define ZMPO_STREAM_UNIT()          \
@@ -748,14 +823,22 @@ __global__ __launch_bounds__(256) void stream_gemm_kernel(
     bool nlast = false;
     if (nvalid) ZMPO_STREAM_FETCH(nks, nlast, nu);
 #pragma unroll
-    for (int kk = 0; kk < KT; kk += 4) {
+    for (int kk = 0; kk < KT; kk += 4 * KG) {
       if (kk >= cks) break;  // uniform
 #pragma unroll
       for (int i = 0; i < PER; ++i) {
         const int blkid = wv + 4 * i;
         if (blkid < NBLK) {
           const int bm = blkid % BM, bn = blkid / BM;
-          acc[i] = Acc<T>::mfma(As[kk + lk][bm * 16 + lr], Bs[kk + lk][bn * 16 + lr], acc[i]);
+          const int am = bm * 16 + lr, bnn = bn * 16 + lr;
+          if constexpr (KG == 1) {
+            acc[i] = Acc<T>::mfma(As[kk + lk][am], Bs[kk + lk][bnn], acc[i]);
+          } else {
+            // k rows past cks hold zeros in both tiles
+            acc[i] = Acc<T>::mfma4(As[kk + lk][am], As[kk + 4 + lk][am], As[kk + 8 + lk][am],
+                                   As[kk + 12 + lk][am], Bs[kk + lk][bnn], Bs[kk + 4 + lk][bnn],
+                                   Bs[kk + 8 + lk][bnn], Bs[kk + 12 + lk][bnn], acc[i]);
+          }
         }
       }
     }
@@ -768,14 +851,15 @@ __global__ __launch_bounds__(256) void stream_gemm_kernel(
           const int bm = blkid % BM, bn = blkid / BM;
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            Abuf[(bm * 16 + Acc<T>::row(lk, r)) * TNv + bn * 16 + lr] = acc[i][r];
+            Abuf[(bm * 16 + Acc<T>::row(lk, r)) * TNv + bn * 16 + lr] =
+                Num<TC>::put((typename Num<TC>::M)acc[i][r]);
           acc[i] = accv{0, 0, 0, 0};
         }
       }
       __syncthreads();
       const zmpo_sunit& un = su[cu];
       if (un.slot >= 0) {
-        T* __restrict__ w = work + (int64_t)un.slot * (TMv * TNv);
+        TC* __restrict__ w = work + (int64_t)un.slot * (TMv * TNv);
         for (int e = tid; e < TMv * TNv; e += 256) w[e] = Abuf[e];
       } else {
         const int mv = min(TMv, un.M - un.m0);
@@ -786,8 +870,7 @@ __global__ __launch_bounds__(256) void stream_gemm_kernel(
             const int mi = un.m0 + row, ni = un.n0 + col;
             const int64_t off = un.c_off + (int64_t)(mi / un.c_mlo) * un.c_smhi +
                                 (int64_t)(mi % un.c_mlo) * un.c_smlo + (int64_t)ni * un.c_sn;
-            const T v = Abuf[e];
-            C[off] = beta ? C[off] + v : v;
+            store_acc(C + off, Num<TC>::get(Abuf[e]), beta);
           }
         }
       }
@@ -806,51 +889,51 @@ __global__ __launch_bounds__(256) void stream_gemm_kernel(
 #undef ZMPO_STREAM_PAIR
 }
 
-template <typename T, int TMv, int TNv, int KT>
+template <typename T, typename TC, int TMv, int TNv, int KT>
 int launch_stream(const zmpo_sunit* units, int64_t nunits, int upw, const zmpo_task* tasks,
                   const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
                   const void* A, const void* B, void* C, void* work, int beta, hipStream_t s) {
   if (nunits > 0) {
     const int64_t nwg = (nunits + upw - 1) / upw;
-    hipLaunchKernelGGL((stream_gemm_kernel<T, TMv, TNv, KT>), dim3((unsigned)nwg), dim3(256), 0,
-                       s, units, nunits, upw, pairs, (const T*)A, (const T*)B, (T*)C, (T*)work,
-                       beta);
+    hipLaunchKernelGGL((stream_gemm_kernel<T, TC, TMv, TNv, KT>), dim3((unsigned)nwg), dim3(256),
+                       0, s, units, nunits, upw, pairs, (const T*)A, (const T*)B, (TC*)C,
+                       (TC*)work, beta);
     int rc = check_hip(hipGetLastError(), "stream_gemm launch");
     if (rc) return rc;
   }
   if (ngroups > 0) {
-    hipLaunchKernelGGL((shaped_reduce_kernel<T, TMv, TNv>), dim3((unsigned)ngroups), dim3(256), 0,
-                       s, tasks, groups, ngroups, (const T*)work, (T*)C, beta);
+    hipLaunchKernelGGL((shaped_reduce_kernel<TC, TMv, TNv>), dim3((unsigned)ngroups), dim3(256), 0,
+                       s, tasks, groups, ngroups, (const TC*)work, (TC*)C, beta);
     return check_hip(hipGetLastError(), "stream_reduce launch");
   }
   return ZMPO_OK;
 }
 
-template <typename T, int KT>
+template <typename T, typename TC, int KT>
 int dispatch_stream_kt(int shape, const zmpo_sunit* units, int64_t nunits, int upw,
                        const zmpo_task* tasks, const zmpo_rgroup* groups, int64_t ngroups,
                        const zmpo_pair* pairs, const void* A, const void* B, void* C, void* work,
                        int beta, hipStream_t s) {
   switch (shape) {
-    case ZMPO_TILE_64x16: return launch_stream<T, 64, 16, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
-    case ZMPO_TILE_128x16: return launch_stream<T, 128, 16, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
-    case ZMPO_TILE_32x32: return launch_stream<T, 32, 32, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
-    case ZMPO_TILE_64x32: return launch_stream<T, 64, 32, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
-    case ZMPO_TILE_16x16: return launch_stream<T, 16, 16, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
-    case ZMPO_TILE_64x64: return launch_stream<T, 64, 64, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_64x16: return launch_stream<T, TC, 64, 16, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_128x16: return launch_stream<T, TC, 128, 16, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_32x32: return launch_stream<T, TC, 32, 32, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_64x32: return launch_stream<T, TC, 64, 32, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_16x16: return launch_stream<T, TC, 16, 16, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_TILE_64x64: return launch_stream<T, TC, 64, 64, KT>(units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
     default: return set_error(ZMPO_ERR_ARG, "streamgemm: unknown tile shape");
   }
 }
 
-template <typename T>
+template <typename T, typename TC = T>
 int dispatch_stream(int shape, int kt, const zmpo_sunit* units, int64_t nunits, int upw,
                     const zmpo_task* tasks, const zmpo_rgroup* groups, int64_t ngroups,
                     const zmpo_pair* pairs, const void* A, const void* B, void* C, void* work,
                     int beta, hipStream_t s) {
   if (kt == 16)
-    return dispatch_stream_kt<T, 16>(shape, units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    return dispatch_stream_kt<T, TC, 16>(shape, units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
   if (kt == 32)
-    return dispatch_stream_kt<T, 32>(shape, units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+    return dispatch_stream_kt<T, TC, 32>(shape, units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
   return set_error(ZMPO_ERR_ARG, "streamgemm: kt must be 16 or 32");
 }
 
@@ -1120,6 +1203,10 @@ extern __shared__ __align__(16) unsigned char chain_lds[];
 #ifndef CHAIN_KS_F32
 #define CHAIN_KS_F32 CHAIN_KS
 #endif
+// BF16: whole MFMA k blocks (4 k-steps each) per slice
+#ifndef CHAIN_KS_BF16
+#define CHAIN_KS_BF16 8
+#endif
 template <typename T>
 struct ChainDepth {
   static constexpr int value = CHAIN_KS;
@@ -1128,12 +1215,33 @@ template <>
 struct ChainDepth<float> {
   static constexpr int value = CHAIN_KS_F32;
 };
+template <>
+struct ChainDepth<bf16s> {
+  static constexpr int value = CHAIN_KS_BF16;
+  static_assert(CHAIN_KS_BF16 % 4 == 0, "BF16 chained slices hold whole MFMA k blocks");
+};
 
 template <typename T, int RB>
 struct ChainSlice {
   static constexpr int KS = ChainDepth<T>::value;
   T a[KS][RB];
   T b[KS][2];
+};
+// BF16 operands held one per 32-bit register (arrays of 16-bit elements
+// were kept in scratch memory by the compiler)
+template <int RB>
+struct ChainSlice<bf16s, RB> {
+  static constexpr int KS = ChainDepth<bf16s>::value;
+  struct R {
+    unsigned v;
+    __device__ R& operator=(bf16s x) {
+      v = x.x;
+      return *this;
+    }
+    __device__ operator bf16s() const { return bf16s{(unsigned short)v}; }
+  };
+  R a[KS][RB];
+  R b[KS][2];
 };
 
 // The fields of a pair that the chained stages use (uniform per wave).
@@ -1190,6 +1298,11 @@ __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPai
         const int64_t ko = (int64_t)min(k0 + 4 * s + lk, pr.K - 1) * bk.a_sk;
 #pragma unroll
         for (int i = 0; i < RB; ++i) sl.a[s][i] = ab[ro[i] + ko];
+      } else if (Acc<T>::KG > 1) {
+        // k-steps past K inside an MFMA k block contribute zero
+        sl.b[s][0] = sl.b[s][1] = Num<T>::zero();
+#pragma unroll
+        for (int i = 0; i < RB; ++i) sl.a[s][i] = Num<T>::zero();
       }
     }
     return;
@@ -1204,6 +1317,10 @@ __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPai
       const T* __restrict__ as = a0 + 4 * s * LD;
 #pragma unroll
       for (int i = 0; i < RB; ++i) sl.a[s][i] = as[lane_a + i * 16];
+    } else if (Acc<T>::KG > 1) {
+      sl.b[s][0] = sl.b[s][1] = Num<T>::zero();
+#pragma unroll
+      for (int i = 0; i < RB; ++i) sl.a[s][i] = Num<T>::zero();
     }
   }
 }
@@ -1281,13 +1398,21 @@ struct ChainWalk {
     return true;
   }
   __device__ __forceinline__ void mfma(const ChainSlice<T, RB>& sl, int ks) {
+    constexpr int KG = Acc<T>::KG;
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
+    for (int s = 0; s < KS; s += KG) {
       if (s >= ks) break;  // uniform
 #pragma unroll
       for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) acc[i][j] = Acc<T>::mfma(sl.a[s][i], sl.b[s][j], acc[i][j]);
+        for (int j = 0; j < NJ; ++j) {
+          if constexpr (KG == 1)
+            acc[i][j] = Acc<T>::mfma(sl.a[s][i], sl.b[s][j], acc[i][j]);
+          else
+            acc[i][j] = Acc<T>::mfma4(sl.a[s][i], sl.a[s + 1][i], sl.a[s + 2][i], sl.a[s + 3][i],
+                                      sl.b[s][j], sl.b[s + 1][j], sl.b[s + 2][j], sl.b[s + 3][j],
+                                      acc[i][j]);
+        }
     }
   }
   __device__ __forceinline__ void flush(int ct, int cn0, int cN, const int (&mb)[RB][4],
@@ -1304,11 +1429,7 @@ struct ChainWalk {
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
               const int n = cn0 + j * 16 + lr;
-              if (n < cN) {
-                T* q = cb + ro + (int64_t)n * tk.c_sn;
-                const T v = acc[i][j][r];
-                *q = beta ? *q + v : v;
-              }
+              if (n < cN) store_acc(cb + ro + (int64_t)n * tk.c_sn, acc[i][j][r], beta);
             }
           }
         }
@@ -1323,7 +1444,8 @@ struct ChainWalk {
         for (int i = 0; i < RB; ++i)
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            dst[(co + n) * LD + i * 16 + Acc<T>::row(lk, r)] = acc[i][j][r];
+            dst[(co + n) * LD + i * 16 + Acc<T>::row(lk, r)] =
+                Num<T>::put((typename Num<T>::M)acc[i][j][r]);
       }
     }
 #pragma unroll
@@ -1420,7 +1542,7 @@ __global__ __launch_bounds__(256) void chain_gemm_kernel(
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
           const int k = (e0 + i * 64 + lane) / TMv;
-          v[i] = (ok0 && k < bk.K) ? a[(int64_t)k * bk.a_sk] : (T)0;
+          v[i] = (ok0 && k < bk.K) ? a[(int64_t)k * bk.a_sk] : Num<T>::zero();
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -1578,6 +1700,12 @@ __global__ void precond_kernel(T* __restrict__ r, const double* __restrict__ dia
     r[i] = (T)((double)r[i] / (fabs(diag[i] - e) + shift));
 }
 
+// value conversion between storage types (FP64 -> BF16 rounds through FP32)
+template <typename TD, typename TS>
+__device__ __forceinline__ TD cvt(TS v) {
+  return Num<TD>::put((typename Num<TD>::M)Num<TS>::get(v));
+}
+
 // dst[i] = src[idx[i]] (idx < 0 -> 0)
 template <typename TD, typename TS>
 __global__ void gather_kernel(TD* __restrict__ dst, const TS* __restrict__ src,
@@ -1585,7 +1713,7 @@ __global__ void gather_kernel(TD* __restrict__ dst, const TS* __restrict__ src,
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t j = idx[i];
-    dst[i] = j >= 0 ? (TD)src[j] : (TD)0;
+    dst[i] = j >= 0 ? cvt<TD>(src[j]) : Num<TD>::zero();
   }
 }
 
@@ -1595,14 +1723,14 @@ __global__ void scatter_kernel(TD* __restrict__ dst, const TS* __restrict__ src,
                                const int64_t* __restrict__ idx, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    dst[idx[i]] = (TD)src[i];
+    dst[idx[i]] = cvt<TD>(src[i]);
 }
 
 template <typename TD, typename TS>
 __global__ void convert_kernel(TD* __restrict__ dst, const TS* __restrict__ src, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = (TD)src[i];
+    dst[i] = cvt<TD>(src[i]);
 }
 
 inline unsigned grid_for(int64_t n, int threads) {
@@ -1779,6 +1907,10 @@ int zmpo_shapedgemm(int dtype, int shape, const zmpo_task* tasks, const zmpo_uni
     return dispatch_shaped<double>(shape, tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
   if (dtype == ZMPO_F32)
     return dispatch_shaped<float>(shape, tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+  if (dtype == ZMPO_BF16)
+    return dispatch_shaped<bf16s>(shape, tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+  if (dtype == ZMPO_BF16_F32)
+    return dispatch_shaped<bf16s, float>(shape, tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
   return set_error(ZMPO_ERR_ARG, "shapedgemm: unsupported dtype");
 }
 
@@ -1839,6 +1971,9 @@ static int chaingemm(bool n16, int dtype, int rb, int atile, int kcols_max, int 
   } else if (dtype == ZMPO_F32) {
     if (rb == 1) ZMPO_CHAIN(float, 1);
     if (rb == 2) ZMPO_CHAIN(float, 2);
+  } else if (dtype == ZMPO_BF16) {
+    if (rb == 1) ZMPO_CHAIN(bf16s, 1);
+    if (rb == 2) ZMPO_CHAIN(bf16s, 2);
   }
 #undef ZMPO_CHAIN
   return set_error(ZMPO_ERR_ARG, "chaingemm: unsupported dtype or row-block count");
@@ -1879,6 +2014,10 @@ int zmpo_streamgemm(int dtype, int shape, int kt, const zmpo_sunit* units, int64
     return dispatch_stream<double>(shape, kt, units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
   if (dtype == ZMPO_F32)
     return dispatch_stream<float>(shape, kt, units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+  if (dtype == ZMPO_BF16)
+    return dispatch_stream<bf16s>(shape, kt, units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
+  if (dtype == ZMPO_BF16_F32)
+    return dispatch_stream<bf16s, float>(shape, kt, units, nunits, upw, tasks, groups, ngroups, pairs, A, B, C, work, beta, s);
   return set_error(ZMPO_ERR_ARG, "streamgemm: unsupported dtype");
 }
 
@@ -1889,7 +2028,8 @@ int zmpo_streamgemm_static(int dtype, int shape, int kt, const zmpo_sunit* units
                            const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
                            const void* A, const void* B, void* C, int64_t work_elems, int beta,
                            void* stream) {
-  const int64_t esz = dtype == ZMPO_F64 ? 8 : dtype == ZMPO_F32 ? 4 : 0;
+  const int64_t esz = dtype == ZMPO_F64 ? 8 : (dtype == ZMPO_F32 || dtype == ZMPO_BF16_F32) ? 4
+                                             : dtype == ZMPO_BF16 ? 2 : 0;
   if (!esz) return set_error(ZMPO_ERR_ARG, "streamgemm_static: unsupported dtype");
   if (work_elems < 0 || (ngroups > 0 && work_elems == 0))
     return set_error(ZMPO_ERR_ARG, "streamgemm_static: work_elems must cover the split-K slots");
@@ -2006,15 +2146,21 @@ int zmpo_convert(int dtype_dst, void* dst, int dtype_src, const void* src, int64
   if (n <= 0) return ZMPO_OK;
   if (!dst || !src) return set_error(ZMPO_ERR_ARG, "convert: null pointer");
   hipStream_t s = (hipStream_t)stream;
-  if (dtype_dst == ZMPO_F32 && dtype_src == ZMPO_F64)
-    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(n, 256)), dim3(256), 0, s,
-                       (float*)dst, (const double*)src, n);
-  else if (dtype_dst == ZMPO_F64 && dtype_src == ZMPO_F32)
-    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n, 256)), dim3(256), 0, s,
-                       (double*)dst, (const float*)src, n);
-  else
-    return set_error(ZMPO_ERR_ARG, "convert: unsupported dtype pair");
-  return check_hip(hipGetLastError(), "convert");
+  const unsigned g = grid_for(n, 256);
+#define ZMPO_CONV(CD, TD, CS, TS)                                                             \
+  if (dtype_dst == CD && dtype_src == CS) {                                                   \
+    hipLaunchKernelGGL((convert_kernel<TD, TS>), dim3(g), dim3(256), 0, s, (TD*)dst,          \
+                       (const TS*)src, n);                                                    \
+    return check_hip(hipGetLastError(), "convert");                                           \
+  }
+  ZMPO_CONV(ZMPO_F32, float, ZMPO_F64, double)
+  ZMPO_CONV(ZMPO_F64, double, ZMPO_F32, float)
+  ZMPO_CONV(ZMPO_BF16, bf16s, ZMPO_F64, double)
+  ZMPO_CONV(ZMPO_BF16, bf16s, ZMPO_F32, float)
+  ZMPO_CONV(ZMPO_F64, double, ZMPO_BF16, bf16s)
+  ZMPO_CONV(ZMPO_F32, float, ZMPO_BF16, bf16s)
+#undef ZMPO_CONV
+  return set_error(ZMPO_ERR_ARG, "convert: unsupported dtype pair");
 }
 
 }  // extern "C"

@@ -3,6 +3,8 @@
 All numerics run in libzmpo kernels; torch only owns the allocations and the
 current HIP stream.
 """
+import os
+
 import numpy as np
 
 _torch = None
@@ -11,6 +13,11 @@ _torch = None
 def torch():
     global _torch
     if _torch is None:
+        # environments of different sizes are allocated and freed every
+        # step: growable segments keep torch's cached-but-free memory
+        # usable for the next sizes (read when the allocator starts, so it
+        # applies unless the process set it or allocated device memory first)
+        os.environ.setdefault('PYTORCH_ALLOC_CONF', 'expandable_segments:True')
         import torch as t
         _torch = t
     return _torch
@@ -30,7 +37,9 @@ def device():
     return torch().device('cuda')
 
 
-TORCH_DTYPES = {0: 'float64', 1: 'float32'}
+# storage of the kernel dtype codes: 0 FP64, 1 FP32, 2 BF16 (ZMPO_BF16),
+# 3 FP32 (the output / workspace side of ZMPO_BF16_F32)
+TORCH_DTYPES = {0: 'float64', 1: 'float32', 2: 'bfloat16', 3: 'float32'}
 
 
 def tdtype(code):
@@ -54,7 +63,9 @@ def upload(arr, code=None):
     a = np.ascontiguousarray(arr)
     if code is not None:
         a = a.astype(np.float64 if code == 0 else np.float32)
-    return t.from_numpy(a).to(device())
+    out = t.from_numpy(a).to(device())
+    # BF16: rounded to nearest even on the device
+    return out.to(t.bfloat16) if code == 2 else out
 
 
 def upload_bytes(struct_arr):
@@ -85,4 +96,7 @@ def sync():
 
 
 def download(x):
-    return x.detach().cpu().numpy()
+    x = x.detach()
+    if x.dtype == torch().bfloat16:
+        x = x.float()
+    return x.cpu().numpy()

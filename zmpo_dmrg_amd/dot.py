@@ -106,7 +106,7 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
         solver.neig = neig
         solver.diag = dv.upload(diag_h, 0)
         psi_lo = dv.empty(ndim0, dtype) if dtype else None
-        sig_lo = dv.empty(ndim0, dtype) if dtype else None
+        sig_lo = dv.empty(ndim0, kernelQt.sigma_dtype(dtype)) if dtype else None
 
         graphs = {}
 
@@ -162,7 +162,7 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
             # them so that only the FP64 environments stay resident
             graphs.pop(dtype, None)
             for env in (fL, fR):
-                env._buf32 = None
+                env.drop_low()
             fine = EigenSolver()
             fine.crit_e, fine.crit_vec = dmrg.crit_e, dmrg.crit_vec
             fine.ndim, fine.neig, fine.diag = ndim0, neig, solver.diag

@@ -49,6 +49,7 @@ SIGMA_STAGE1 = os.environ.get('ZMPO_STAGE1', 'shaped')   # grouped-GEMM kernel o
 # K = 24, D = 500
 SIGMA_STAGE4 = os.environ.get('ZMPO_STAGE4', 'stream')
 STAGE4_KT = {0: 16, 1: 32}
+STAGE4_KT.update({2: 32, 3: 32})
 # tile shapes (qtensor.SHAPES ids) of the L x psi and x R launches ('' = cost
 # model).  32x32 for both: a sigma block's right dimension (~26 at D = 1000)
 # fits one tile width, and the small LDS footprint keeps 8 workgroups per CU
@@ -88,22 +89,29 @@ class EnvStore:
 
     def __init__(self, meta, buf, off):
         self.meta, self.buf, self.off = meta, buf, np.asarray(off, dtype=np.int64)
-        self._buf32 = None
+        self._lo = {}
 
     @property
     def nops(self):
         return len(self.meta)
 
     def buffer(self, dtype=0):
+        """The values in one precision (0 FP64, 1 FP32, 2 BF16); lower
+        precisions are converted once and kept until drop_low()."""
         if isinstance(self.buf, np.ndarray):
             self.buf = dv.upload(self.buf, 0)
         if dtype == 0:
             return self.buf
-        if self._buf32 is None:
-            self._buf32 = dv.empty(self.buf.numel(), 1)
-            _lib.check(_lib.lib().zmpo_convert(1, dv.ptr(self._buf32), 0, dv.ptr(self.buf),
+        if dtype not in self._lo:
+            b = dv.empty(self.buf.numel(), dtype)
+            _lib.check(_lib.lib().zmpo_convert(dtype, dv.ptr(b), 0, dv.ptr(self.buf),
                                                self.buf.numel(), dv.stream()))
-        return self._buf32
+            self._lo[dtype] = b
+        return self._lo[dtype]
+
+    def drop_low(self):
+        """Release the low-precision copies."""
+        self._lo = {}
 
     def op_tensor(self, p):
         """QTensor of op p with a host copy of its values (tests, export)."""
@@ -166,6 +174,17 @@ def _merge_shared_many(items, nout):
     tasks['p_begin'] = np.concatenate([[0], np.cumsum(counts)[:-1]])
     tasks['p_end'] = tasks['p_begin'] + counts
     return tasks, p[order]
+
+
+def out_dtype(dtype):
+    """Kernel dtype code of the sigma launch into sigma: BF16 operands
+    accumulate into an FP32 sigma (ZMPO_BF16_F32)."""
+    return 3 if dtype == 2 else dtype
+
+
+def sigma_dtype(dtype):
+    """Storage code of sigma for a sigma computed with operands of dtype."""
+    return 1 if dtype == 2 else dtype
 
 
 def _chunks(sizes, budget):
@@ -324,8 +343,10 @@ class SigmaProgram:
         return b
 
     def apply(self, mpo, lenv, renv, psi, sigma, dtype=0, fused=None, overlap=None):
-        """sigma = H psi (device arrays of length ndim0, same dtype).  overlap=False
-        keeps every launch on the current stream (as graph capture needs)."""
+        """sigma = H psi (device arrays of length ndim0, same dtype; for dtype
+        2, BF16 operands with FP32 accumulation, psi is BF16 and sigma FP32).
+        overlap=False keeps every launch on the current stream (as graph
+        capture needs)."""
         fused = self.fused if fused is None else fused
         overlap = OVERLAP if overlap is None else overlap
         t1, t2, t3 = self.buffers(dtype, mid=not fused)
@@ -349,7 +370,7 @@ class SigmaProgram:
                 run_program(st[1], t1, Wi, t2, dtype)
                 run_program(st[2], t2, Wj, t3, dtype)
             if st[3].ntasks:
-                run_program(st[3], t3, R, sigma, dtype, beta=0 if first else 1)
+                run_program(st[3], t3, R, sigma, out_dtype(dtype), beta=0 if first else 1)
                 first = False
         return sigma
 
@@ -383,7 +404,7 @@ class SigmaProgram:
             done.record(main)
             free[c % 2] = done
             if st[3].ntasks:
-                run_program(st[3], t3, R, sigma, dtype, beta=0 if first else 1,
+                run_program(st[3], t3, R, sigma, out_dtype(dtype), beta=0 if first else 1,
                             stream=main.cuda_stream)
                 first = False
         if first:
@@ -408,7 +429,7 @@ class SigmaGraph:
         n = prog.ndim0
         self.prog, self.dtype = prog, dtype
         self.x = dv.zeros(n, dtype)
-        self.y = dv.zeros(n, dtype)
+        self.y = dv.zeros(n, sigma_dtype(dtype))
         self.empty = not prog.stages or all(st[3].ntasks == 0 for st in prog.stages)
         if self.empty:
             self.graph = None
@@ -935,7 +956,7 @@ def HVecQt(vec0, info, ndim0, prjmap, ifile=0, ioff=0):
     dtype = getattr(dmrg, 'dtype', 0)
     host = isinstance(vec0, np.ndarray)
     psi = dv.upload(vec0, dtype) if host else vec0
-    sig = dv.empty(ndim0, dtype)
+    sig = dv.empty(ndim0, sigma_dtype(dtype))
     prog.apply(dmrg.fhop, fL, fR, psi, sig, dtype)
     return dv.download(sig).astype(np.float64) if host else sig
 

@@ -432,6 +432,6 @@ class MPOStore:
             if self.buf[i] is None:
                 self.buf[i] = dv.upload(self.host[i], 0)
             return self.buf[i]
-        if i not in self._buf32:
-            self._buf32[i] = dv.upload(self.host[i], 1)
-        return self._buf32[i]
+        if (i, dtype) not in self._buf32:
+            self._buf32[(i, dtype)] = dv.upload(self.host[i], dtype)
+        return self._buf32[(i, dtype)]
