@@ -80,7 +80,9 @@ class MPODMRG:
         self.nsigma = 0
         self.sigma_flops = 0
         self.niters = []
-        self.timing = {'solve': 0.0, 'renorm': 0.0, 'decim': 0.0}
+        # wall seconds: 'solve' (the two-site step: 'diag', 'sigma_plan',
+        # 'davidson', 'decimation', 'guess') and 'renorm'
+        self.timing = {'solve': 0.0, 'renorm': 0.0}
         # keep_envs=False: a half-sweep releases the environments it will
         # not read again (the next half-sweep rebuilds them before use), so
         # about one environment per bond is resident instead of two

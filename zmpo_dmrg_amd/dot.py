@@ -18,6 +18,7 @@ unstable argsort (mpo_dmrg_qparser.py:220-233, 259) and cannot be
 reproduced.
 """
 import os
+import time
 
 import numpy as np
 
@@ -90,14 +91,25 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
     setupQlst(dmrg, isite)
     info = [dmrg, isite, ncsite, flst, status, ifsym]
     civecs, eigs, nmvps = [], [], 0
+    tm = dmrg.__dict__.setdefault('timing', {})
+
+    def lap(key, t0):
+        """Accumulate wall time since t0 under key (device work finished)."""
+        dv.sync()
+        t = time.time()
+        tm[key] = tm.get(key, 0.0) + t - t0
+        return t
     for key, neig in dmrg.qsectors.items():
         setupQtmpCI(dmrg, key)
         prjmap = dmrg.qtmp.prjmap(dmrg.idlstCI)
         ndim0 = dmrg.qtmp.size_allowed
         if ndim0 == 0:
             raise RuntimeError('no basis in the symmetry sector %s' % (key,))
+        t0 = time.time()
         diag_h = kernelQt.HDiagQt(info, ndim0, prjmap)
+        t0 = lap('diag', t0)
         prog = kernelQt._sigma_program(dmrg, isite, fL, fR)
+        t0 = lap('sigma_plan', t0)
         dtype = getattr(dmrg, 'dtype', 0)
         solver = EigenSolver()
         solver.crit_e = dmrg.crit_e
@@ -169,6 +181,7 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
             fine.HVec = hvec64
             eig, vec, nmvp64 = fine.solve_iter(dv.download(vec).reshape(neig, ndim0))
             nmvp += nmvp64
+        t0 = lap('davidson', t0)
         vec_h = dv.download(vec)
         civec = np.zeros((neig, dmrg.dims[3]))
         civec[:, prjmap] = vec_h
@@ -177,8 +190,11 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
         nmvps += nmvp
         dmrg.niters.append(solver.niter)
     civecs = np.vstack(civecs)
+    t0 = time.time()
     sigs, dwts, qred, site, srotR = decimation(dmrg, civecs, isite, status, info)
+    t0 = lap('decimation', t0)
     genGuess(dmrg, isite, actlst, civecs, srotR, status)
+    lap('guess', t0)
     return nmvps, np.array(eigs), sigs, dwts, qred, site, srotR
 
 

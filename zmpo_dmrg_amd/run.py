@@ -72,6 +72,8 @@ def main(argv=None):
         out['nsigma'].append(d.nsigma)
         print('[run] sweep %d D=%d tol=%.0e noise=%.0e E=%.10f %.1fs sigma %d peak %.1f GiB' % (
             isweep, d.Dmax, d.crit_e, d.noise, eav, dt, d.nsigma, gib(True)), flush=True)
+        print('[run]   timing so far: %s' % ' '.join('%s %.1f' % kv for kv in sorted(
+            d.timing.items())), flush=True)
         if args.save:
             mpsio.save(args.save, {i: d.lmps[i] for i in range(K)},
                        [np.asarray(d.qnuml[i]) for i in range(K + 1)])
