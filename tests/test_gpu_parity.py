@@ -172,27 +172,39 @@ def test_device_dmrg_matches_oracle_fixture(gpu, name):
         assert np.array_equal(np.asarray(d.qnumr[i]), z['qnumr%d' % i]), i
 
 
-def test_fp32_sigma_with_fp64_correction_matches_fp64(gpu):
-    """Precision form: Davidson with FP32-MFMA sigma, then restarted from its
-    solution with FP64 sigma (residual correction).  K=6, N=6 at D=64 keeps
+@pytest.mark.parametrize('mode,lo', [('correction', 1), ('correction', 2), ('restart', 1)])
+def test_fp32_sigma_with_fp64_correction_matches_fp64(gpu, mode, lo):
+    """Precision form: low-precision sigma (FP32 or BF16 operands) with FP64
+    correction -- 'correction': outer FP64 sigmas around anchored inner
+    low-precision Davidson solves; 'restart': a low-precision Davidson
+    restarted from its solution with FP64 sigma.  K=6, N=6 at D=64 keeps
     every bond untruncated, so the converged energy is the FCI energy: the
-    FP32 run must match the FP64 run and FCI to 1e-6 Ha (the target; the
-    restart makes it an FP64 Ritz value, observed ~1e-10), with both the
-    FP32 and the FP64 phase taking part.  (Per-step comparisons against a
-    truncated FP64 run are not meaningful: RDM weight ties at the 1e-9
-    level can pick a different truncated space.)"""
+    precision run must match the FP64 run and FCI to 1e-6 Ha (the target;
+    the reported value is an FP64 Rayleigh quotient), with both the
+    low-precision and the FP64 sigma taking part.  (Per-step comparisons
+    against a truncated FP64 run are not meaningful: RDM weight ties at the
+    1e-9 level can pick a different truncated space.)"""
     from oracle import fci
+    from zmpo_dmrg_amd import dot as pdot
     K = 6
     i1, i2 = gen_integrals(K, seed=2)
     h, eri = gen_spatial(K, seed=2)
     efci = fci.fci_energy(h, eri, K, 0.0)[0]
     out = {}
-    for dtype in (0, 1):
-        d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=64, tol=1e-10, dtype=dtype)
-        out[dtype] = (pdmrg.run(d, 3)[-1], d)
-    e64, e32 = out[0][0], out[1][0]
+    saved = pdot.PRECISION_MODE
+    pdot.PRECISION_MODE = mode
+    try:
+        for dtype in (0, lo):
+            d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=64, tol=1e-10, dtype=dtype)
+            out[dtype] = (pdmrg.run(d, 3)[-1], d)
+    finally:
+        pdot.PRECISION_MODE = saved
+    e64, e32 = out[0][0], out[lo][0]
+    print('[prec] %s dtype %d: |E - E64| = %.2e, sigma low %d / FP64 %d' % (
+        mode, lo, abs(e32 - e64), out[lo][1].nsigma, getattr(out[lo][1], 'nsigma64', 0)),
+        flush=True)
     assert abs(e64 - efci) < 1e-8, (e64, efci)
     assert abs(e32 - e64) < 1e-6, (e32, e64)
-    d = out[1][1]
+    d = out[lo][1]
     n64 = getattr(d, 'nsigma64', 0)
     assert d.nsigma > 0 and n64 > 0, (d.nsigma, n64)   # both phases ran

@@ -13,11 +13,14 @@ _torch = None
 def torch():
     global _torch
     if _torch is None:
-        # environments of different sizes are allocated and freed every
-        # step: growable segments keep torch's cached-but-free memory
-        # usable for the next sizes (read when the allocator starts, so it
-        # applies unless the process set it or allocated device memory first)
-        os.environ.setdefault('PYTORCH_ALLOC_CONF', 'expandable_segments:True')
+        # environments and intermediates of many sizes (up to tens of GB)
+        # are allocated and freed every step: cached blocks above 256 MB are
+        # never split, so a large request either reuses a whole block or
+        # makes torch return its free cached blocks to the device before
+        # allocating (growable segments are not available on this
+        # platform).  Read when the allocator starts: it applies unless the
+        # process set it or allocated device memory first.
+        os.environ.setdefault('PYTORCH_ALLOC_CONF', 'max_split_size_mb:256')
         import torch as t
         _torch = t
     return _torch

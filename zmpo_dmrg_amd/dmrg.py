@@ -103,8 +103,8 @@ class MPODMRG:
 def restart_from_left_mps(dmrg, sites, qnuml):
     """Continue from a left-canonical MPS (the state after an 'L' half-sweep;
     the last site holds the normalised coefficients): its sites and bond
-    QNs replace the product state, every left environment is rebuilt
-    (initOps, mpo_dmrg_util.py:193-214) and no guess is carried over."""
+    QNs replace the product state and every left environment is rebuilt
+    (initOps, mpo_dmrg_util.py:193-214)."""
     K = dmrg.nsite
     for i in range(K):
         dmrg.lmps[i] = sites[i]
@@ -112,7 +112,11 @@ def restart_from_left_mps(dmrg, sites, qnuml):
         dmrg.qnuml[i] = np.asarray(qnuml[i])
     dmrg.envL, _ = kernelQt.genHops(dmrg.fhop, dmrg.lmps, 'L')
     dmrg.envR = {K: kernelQt.boundary_store(dmrg.nops)}
-    dmrg.psi0 = None
+    # the first step of the next 'R' half-sweep (sites K-2, K-1) starts from
+    # the MPS's own two-site wavefunction
+    a = dmrg.lmps[K - 2].toDenseTensor()
+    b = dmrg.lmps[K - 1].toDenseTensor()
+    dmrg.psi0 = np.tensordot(a, b, axes=([2], [0])).reshape(1, -1)
 
 
 def sweep_updateSite(dmrg, isite, qred, site, status):

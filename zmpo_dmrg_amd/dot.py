@@ -159,7 +159,12 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
         v0 = None
         if dmrg.psi0 is not None:
             v0 = np.ascontiguousarray(dmrg.psi0[:neig][:, prjmap])
-        if dtype:
+        if dtype and neig == 1 and PRECISION_MODE == 'correction':
+            eig, vec, nmvp = _defect_correction(dmrg, solver, hvec, hvec64, v0, ndim0, dtype)
+            for env in (fL, fR):
+                env.drop_low()
+            graphs.pop(dtype, None)
+        elif dtype:
             # low-precision sigma: converge to residual 1e-4 (what FP32 sigma
             # resolves with margin), then
             # restart Davidson from that solution with FP64 sigma (residual
@@ -167,8 +172,10 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
             solver.crit_vec = max(dmrg.crit_vec, getattr(dmrg, 'crit_vec_lo', 1e-4))
             solver.crit_e = max(dmrg.crit_e, getattr(dmrg, 'crit_e_lo', 1e-7))
             solver.herm_tol = 1e-5
-        eig, vec, nmvp = solver.solve_iter(v0)
-        if dtype:
+            eig, vec, nmvp = solver.solve_iter(v0)
+        else:
+            eig, vec, nmvp = solver.solve_iter(v0)
+        if dtype and not (neig == 1 and PRECISION_MODE == 'correction'):
             # the low-precision copies of this step's environments are not
             # needed again (the next step at this bond rebuilds them): drop
             # them so that only the FP64 environments stay resident
@@ -182,6 +189,9 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
             eig, vec, nmvp64 = fine.solve_iter(dv.download(vec).reshape(neig, ndim0))
             nmvp += nmvp64
         t0 = lap('davidson', t0)
+        # the sigma intermediates are not needed before the next step's
+        # program: free them ahead of the decimation's own buffers
+        prog._buffers.clear()
         vec_h = dv.download(vec)
         civec = np.zeros((neig, dmrg.dims[3]))
         civec[:, prjmap] = vec_h
@@ -196,6 +206,76 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
     genGuess(dmrg, isite, actlst, civecs, srotR, status)
     lap('guess', t0)
     return nmvps, np.array(eigs), sigs, dwts, qred, site, srotR
+
+
+# Precision form (dmrg.dtype 1 = FP32, 2 = BF16 sigma operands):
+# 'correction' (default): outer FP64 corrections around inner low-precision
+# solves (_defect_correction); 'restart': a low-precision Davidson to
+# residual 1e-4 followed by a full FP64 Davidson from its vector.
+PRECISION_MODE = os.environ.get('ZMPO_PRECISION_MODE', 'correction')
+# inner solves stop when their residual falls below this fraction of the
+# current FP64 residual, or after this many iterations
+INNER_FRAC = 0.05
+INNER_MAXCYCLE = 12
+OUTER_MAX = 8
+
+
+def _defect_correction(dmrg, solver, hvec_lo, hvec64, v0, n, dtype):
+    """Lowest eigenpair of H_eff with FP64 accuracy from mostly low-precision
+    sigma applications (one state).
+
+    Outer iteration k: y = H x_k with FP64 sigma, theta_k = x_k.y (x_k
+    normalised), r_k = y - theta_k x_k.  Stop when |r_k| <= crit_vec or
+    |theta_k - theta_(k-1)| <= crit_e (the Davidson criteria of
+    pdvdson.py:161-188, on FP64 quantities).  Otherwise an inner Davidson
+    with the low-precision sigma starts from x_k, whose first basis vector
+    keeps its exact sigma y (pdvdson.EigenSolver.solve_iter(v0, w0)), and
+    returns x_(k+1).  The operator error of the inner solve enters only
+    through the correction x_(k+1) - x_k, so each outer step shrinks the
+    error by about |H_lo - H| / gap; the reported eigenvalue is the FP64
+    Rayleigh quotient of the returned vector.  Returns (eigs, vec, nmvp)."""
+    from .pdvdson import _VecOps
+    t = dv.torch()
+    ops = _VecOps(n)
+    if v0 is None:
+        d = dv.download(solver.diag) + 1e-12 * np.arange(1, 1 + n) / float(n)
+        x = t.zeros(n, dtype=t.float64, device=dv.device())
+        x[int(np.argsort(d, kind='stable')[0])] = 1.0
+    else:
+        x = dv.upload(np.asarray(v0).reshape(-1), 0)
+    x.mul_(1.0 / ops.norm(x))
+    y = t.empty_like(x)
+    r = t.empty_like(x)
+    nmvp = 0
+    theta_prev = None
+    herm = 1e-5 if dtype == 1 else 2e-2
+    niter = 0
+    for k in range(OUTER_MAX):
+        hvec64(x, y)
+        nmvp += 1
+        theta = float(ops.dots(x.view(1, -1), 1, y)[0])
+        r.copy_(y)
+        r.add_(x, alpha=-theta)
+        rn = ops.norm(r)
+        if rn <= dmrg.crit_vec or (theta_prev is not None
+                                   and abs(theta - theta_prev) <= dmrg.crit_e):
+            break
+        theta_prev = theta
+        inner = EigenSolver()
+        inner.crit_vec = max(dmrg.crit_vec, INNER_FRAC * rn)
+        inner.crit_e = -1.0                   # residual criterion only
+        inner.maxcycle = INNER_MAXCYCLE
+        inner.herm_tol = herm
+        inner.ndim, inner.neig, inner.diag = n, 1, solver.diag
+        inner.HVec = hvec_lo
+        _, vec, nm = inner.solve_iter(x.view(1, -1), w0=y.view(1, -1))
+        nmvp += nm
+        niter += inner.niter
+        x.copy_(vec[0])
+        x.mul_(1.0 / ops.norm(x))
+    solver.niter = niter
+    dmrg.nouter = getattr(dmrg, 'nouter', 0) + k + 1
+    return np.array([theta]), x.view(1, -1), nmvp
 
 
 def canonical_null(u, s2, tol):

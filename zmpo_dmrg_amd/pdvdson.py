@@ -134,7 +134,10 @@ class EigenSolver:
                     ops.project_out(R[j], V[out_start:], nindp)
         return nindp
 
-    def solve_iter(self, v0=None):
+    def solve_iter(self, v0=None, w0=None):
+        """Lowest neig eigenpairs.  w0 (neig = 1 only): the sigma of v0,
+        already known (for instance in a higher precision than HVec): the
+        first basis vector keeps it instead of calling HVec."""
         t = dv.torch()
         n, neig = self.ndim, self.neig
         if neig > n:
@@ -154,15 +157,22 @@ class EigenSolver:
         else:
             V[:neig].copy_(v0 if not isinstance(v0, np.ndarray) else dv.upload(v0, 0).view(neig, n))
         # genOrthoBas (pdvdson.py:335-345)
-        V[0].mul_(1.0 / ops.norm(V[0]))
+        if w0 is not None and neig != 1:
+            raise DavidsonError('a given first sigma needs neig = 1')
+        nrm0 = ops.norm(V[0])
+        V[0].mul_(1.0 / nrm0)
         if neig > 1:
             R2 = V[1:neig].clone()
             nindp = self._ortho(ops, V, 1, R2, list(range(neig - 1)), 1)
             if nindp + 1 != neig:
                 raise DavidsonError('insufficient orthonormal basis')
-        for i in range(neig):
-            self.HVec(V[i], W[i])
-        self.nmvp += neig
+        if w0 is not None:
+            W[0].copy_(w0.reshape(-1))
+            W[0].mul_(1.0 / nrm0)
+        else:
+            for i in range(neig):
+                self.HVec(V[i], W[i])
+            self.nmvp += neig
         nb = neig
         eigs = np.zeros(neig) + 1e3
         vr = None

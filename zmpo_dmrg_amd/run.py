@@ -31,6 +31,11 @@ def main(argv=None):
     ap.add_argument('--budget-gb', type=float, default=64.0)
     ap.add_argument('--save', default=None)
     ap.add_argument('--load', default=None)
+    ap.add_argument('--start-m', type=int, default=None,
+                    help='first bond dimension of the Normal2 ramp (default: the reference rule)')
+    ap.add_argument('--compare', type=int, nargs='*', default=[],
+                    help='after the schedule, one more full sweep (tolerance --tol, no noise) '
+                         'in FP64 and then, from the same state, in each of these dtypes')
     args = ap.parse_args(argv)
     from . import device as dv
     from . import dmrg as pd
@@ -50,6 +55,7 @@ def main(argv=None):
     print('[run] K=%d D=%d dtype=%d set up in %.1fs' % (K, args.D, args.dtype, time.time() - t0),
           flush=True)
     sched = Schedule(maxM=args.D, tol=args.tol)
+    sched.startM = args.start_m
     sched = sched.normal() if args.schedule == 'normal' else sched.fixed(noise=args.noise)
 
     def gib(peak=False):
@@ -80,9 +86,59 @@ def main(argv=None):
 
     pd.run(d, args.sweeps, schedule=sched, log=sweep_log)
     out.update({'K': K, 'D': args.D, 'dtype': args.dtype, 'schedule': args.schedule,
-                'timing': d.timing})
+                'timing': dict(d.timing)})
+    if args.compare:
+        out['compare'] = compare_precisions(d, args, step_log)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def compare_precisions(d, args, step_log):
+    """One full sweep (R then L, Davidson tolerance args.tol, no noise) from
+    the current state in FP64, then the same sweep from the same state in
+    each dtype of args.compare.  Per dtype: sweep seconds, the per-step
+    eigenvalues and their difference from the FP64 sweep (the first step
+    of each run solves the same H_eff; later steps follow each run's own
+    truncations), sigma counts and the phase timing."""
+    import os
+    import tempfile
+    from . import dmrg as pd
+    from . import mpsio
+    K = d.nsite
+    path = os.path.join(tempfile.gettempdir(), 'zmpo_compare_%d.npz' % os.getpid())
+    mpsio.save(path, {i: d.lmps[i] for i in range(K)},
+               [np.asarray(d.qnuml[i]) for i in range(K + 1)])
+    res = {}
+    base = None
+    for dt in [0] + list(args.compare):
+        sites, qnums = mpsio.load(path)
+        pd.restart_from_left_mps(d, sites, qnums)
+        d.dtype = dt
+        d.Dmax, d.crit_e, d.noise = args.D, args.tol, 0.0
+        d.timing = {'solve': 0.0, 'renorm': 0.0}
+        n0, n640 = d.nsigma, getattr(d, 'nsigma64', 0)
+        t0 = time.time()
+        r1 = pd.sweep(d, 'R')
+        r2 = pd.sweep(d, 'L')
+        dt_s = time.time() - t0
+        steps = [float(e[0]) for e in r1[3]] + [float(e[0]) for e in r2[3]]
+        ent = {'sweep_s': dt_s, 'energy': min(r1[2], r2[2]), 'steps': steps,
+               'nsigma_low': d.nsigma - n0 if dt else 0,
+               'nsigma_f64': (getattr(d, 'nsigma64', 0) - n640) if dt else d.nsigma - n0,
+               'timing': dict(d.timing)}
+        if base is None:
+            base = ent
+        else:
+            ent['dE_first_step'] = abs(steps[0] - base['steps'][0])
+            ent['dE_sweep'] = abs(ent['energy'] - base['energy'])
+            ent['speedup'] = base['sweep_s'] / dt_s
+        res[dt] = ent
+        print('[run] compare dtype %d: sweep %.1fs E=%.10f %s' % (
+            dt, dt_s, ent['energy'], ' '.join('%s=%.3g' % (k, v) for k, v in ent.items()
+                                               if k in ('dE_first_step', 'dE_sweep', 'speedup',
+                                                        'nsigma_low', 'nsigma_f64'))), flush=True)
+    os.remove(path)
+    return res
 
 
 if __name__ == '__main__':
