@@ -44,6 +44,8 @@ def build_workload(K, D, seed, dtype, budget, fused=True, extra=None):
     log('MPO K=%d built in %.1fs' % (K, time.time() - t0))
     isite = K // 2 - 1
     ls, rs, ql, qr = synthetic.synthetic_mps(K, K, D, isite - 1, isite + 2, seed=seed)
+    if extra is not None:
+        extra.update(ls=ls, rs=rs, ql=ql, qr=qr)
     log('synthetic MPS: bond dims L %s R %s' % (len(ql[isite]), len(qr[isite + 2])))
     env = kq.boundary_store(mpo.nops)
     ren_flops = 0
@@ -150,6 +152,54 @@ def traffic_per_launch(args, launch):
     return rec.get('bytes_per_launch')
 
 
+def step_window(mpo, isite, L, R, extra, D, dtype, budget, nsteps=2, tol=1e-8, maxcycle=None):
+    """nsteps full two-site steps ('L' direction from the middle bond of the
+    synthetic state): Davidson solve (the first from the lowest diagonal
+    element, the next from the reference's guess), decimation to D, site
+    update and renormalisation, as a sweep does them.  Returns per-step
+    seconds and eigenvalues, sigma counts and the phase timing."""
+    from zmpo_dmrg_amd import device as dv
+    from zmpo_dmrg_amd import dmrg as pd
+    from zmpo_dmrg_amd import kernelQt as kq
+    from zmpo_dmrg_amd import mpo as pmpo
+    from zmpo_dmrg_amd.dot import ci_solver
+    K = mpo.nsite
+    d = pd.MPODMRG.__new__(pd.MPODMRG)
+    d.fhop, d.nsite, d.sbas, d.nops = mpo, K, mpo.sbas, mpo.nops
+    d.opers, d.hpwts = mpo.opers, mpo.hpwts
+    d.qphys = [pmpo.qphys() for _ in range(K)]
+    d.qsectors = {(float(K), 0.0): 1}
+    d.Dmax, d.crit_e, d.crit_vec, d.thresh, d.noise = D, tol, tol, -1e-14, 0.0
+    d.dtype, d.budget, d.psi0, d.keep_envs = dtype, budget, None, True
+    d.nsigma, d.sigma_flops, d.niters = 0, 0, []
+    if maxcycle:
+        d.maxcycle = maxcycle
+    d.timing = {'solve': 0.0, 'renorm': 0.0}
+    d.qnuml, d.qnumr = dict(extra['ql']), dict(extra['qr'])
+    d.lmps, d.rmps = dict(extra['ls']), dict(extra['rs'])
+    d.envL = {isite - 1: L}
+    d.envR = {isite + 2: R, isite + 3: extra['Rprev']}
+    actlst = list(range(isite, isite + nsteps + 1))
+    times, eigs = [], []
+    for n in range(nsteps):
+        i = isite + n
+        flst = [[d.envL[i - 1], d.envR[i + 2]]]
+        dv.sync()
+        t = time.time()
+        _, e, _, _, qred, site, _ = ci_solver(d, i, 2, actlst, flst, 'L')
+        qsite = pd.sweep_updateSite(d, i, qred, site, 'L')
+        flstN = [[None]]
+        kq.renormQt_H00(d, i, 2, flst, flstN, qsite, 'L')
+        d.envL[i] = flstN[0][0]
+        dv.sync()
+        times.append(time.time() - t)
+        eigs.append(float(e[0]))
+        log('window dtype %d step %d: E=%.10f %.2fs' % (dtype, i, eigs[-1], times[-1]))
+    return {'step_s': times, 'eigs': eigs, 'nsigma_low': d.nsigma if dtype else 0,
+            'nsigma_f64': getattr(d, 'nsigma64', 0) if dtype else d.nsigma,
+            'timing': {k: round(v, 3) for k, v in d.timing.items()}}
+
+
 def to_oracle(q):
     """An oracle QTensor with q's tables and values (host copy; values from
     the device layout into the reference's row-major blocks)."""
@@ -216,6 +266,8 @@ def main():
     ap.add_argument('--dtype', choices=['f64', 'f32'], default='f64')
     ap.add_argument('--budget-gb', type=float, default=96.0)
     ap.add_argument('--cpu-ops', type=int, default=2)
+    ap.add_argument('--window', type=int, default=2,
+                    help='full two-site steps timed per precision after the sigma (0: none)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-f32', action='store_true',
                     help='skip the FP32-operand sigma timed after the FP64 one')
@@ -235,9 +287,10 @@ def main():
         pq.KERNEL = args.kernel
     torch = dv.torch()
     dtype = 0 if args.dtype == 'f64' else 1
+    extra = {}
     mpo, isite, st, L, R, prog = build_workload(args.K, args.D, args.seed, dtype,
                                                 int(args.budget_gb * 2 ** 30),
-                                                fused=not args.unfused)
+                                                fused=not args.unfused, extra=extra)
     rng = np.random.default_rng(1)
     psi_h = rng.standard_normal(st.qtmp.size_allowed)
     psi_h /= np.linalg.norm(psi_h)
@@ -321,6 +374,34 @@ def main():
                      'bf16_stage_ms': [round(x, 4) for x in st_ms16]})
         log('BF16 sigma: %.3f ms/step (%.2fx FP64), relative error %.2e, stages %s' % (
             ms16, ms_step / ms16, err16, st_ms16))
+    # full two-site steps (solve, decimation, renormalisation) in each
+    # precision from the same state, after the timed region
+    window = None
+    if args.window > 0 and dtype == 0:
+        prog._buffers.clear()
+        window = {}
+        for dt in (0, 1, 2):
+            window[dt] = step_window(mpo, isite, L, R, extra, args.D, dt,
+                                     int(args.budget_gb * 2 ** 30), args.window)
+        # the first step's H_eff is the same in every run: its lowest
+        # eigenvalue from an FP64 Davidson run to convergence (no iteration
+        # cap) is the value every precision is measured against
+        exact = step_window(mpo, isite, L, R, extra, args.D, 0, int(args.budget_gb * 2 ** 30),
+                            1, maxcycle=400)['eigs'][0]
+        base = window[0]
+        for dt in (0, 1, 2):
+            w = window[dt]
+            w['dE_first_step'] = abs(w['eigs'][0] - exact)
+            w['step_speedup'] = sum(base['step_s']) / sum(w['step_s'])
+        nstep = 2 * (args.K - 1)
+        window = {'steps': args.window, 'tol': 1e-8, 'D': args.D, 'first_step_exact': exact,
+                  'sweep_s_estimate': {str(dt): round(nstep * float(np.mean(window[dt]['step_s'])), 1)
+                                       for dt in window},
+                  'how': 'mean of the window steps (middle bond, the largest blocks) x 2(K-1) '
+                         'steps per sweep; dtype 0 FP64, 1 FP32 + FP64 correction, 2 BF16 + '
+                         'FP64 correction',
+                  'runs': {str(k): v for k, v in window.items()}}
+        log('window: %s' % json.dumps(window['sweep_s_estimate']))
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -344,7 +425,7 @@ def main():
             args.K, args.K, args.D, mpo.nops), 'ndim0': int(st.qtmp.size_allowed),
             'sigma_gflop': prog.flops / 1e9, 'chunks': len(prog.chunks),
             'launches': ' | '.join(names)},
-        'roofline': roof, 'cpu_baseline': cpu, 'precision': prec,
+        'roofline': roof, 'cpu_baseline': cpu, 'precision': prec, 'sweep_window': window,
     }
     print(json.dumps(out), flush=True)
 

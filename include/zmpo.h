@@ -301,6 +301,53 @@ int zmpo_scatter(int dtype_dst, void* dst, int dtype_src, const void* src, const
 int zmpo_convert(int dtype_dst, void* dst, int dtype_src, const void* src, int64_t n,
                  void* stream);
 
+/* Device store: allocation and copies of the buffers the launches read and
+ * write (MPO sites, environments, psi / sigma, intermediates).  No
+ * counterpart in the reference, whose store is HDF5 files (mpo_dmrg_io.py:
+ * 168-206, mpo_dmrg_opt.py:196-238).  zmpo_memcpy returns after host
+ * copies complete; device-to-device copies are ordered on `stream`. */
+#define ZMPO_H2D 0
+#define ZMPO_D2H 1
+#define ZMPO_D2D 2
+int zmpo_alloc(void** out, int64_t bytes);
+int zmpo_free(void* p);
+int zmpo_memcpy(void* dst, const void* src, int64_t bytes, int kind, void* stream);
+
+/* Launch sequences: a whole two-site sigma (HVecQt, source/mpo_dmrg_
+ * kernelQt.py:153-319: L x psi, the chained W[i] x W[i+1] launch, x R per
+ * operator chunk), renormalisation (renormQt_H00, kernelQt.py:633-767) or
+ * diagonal (HDiagQt, kernelQt.py:30-127) recorded once per plan and issued
+ * by one call.  Each op is one zmpo_shapedgemm / zmpo_streamgemm /
+ * zmpo_chaingemm(16) launch whose tables (device pointers, owned by the
+ * caller and kept alive while the sequence is used) are recorded; its
+ * operands are slot numbers bound at run time: A = slots[slot_a], B =
+ * slots[slot_b], the chained launch's second B = slots[slot_b2], C =
+ * slots[slot_c], split-K workspace = slots[slot_work] (-1: none).  out_f32:
+ * run this op as ZMPO_BF16_F32 when the sequence runs as ZMPO_BF16 (the
+ * sigma output).  For chains, groups / blocks hold the zmpo_cgroup /
+ * zmpo_cblock arrays and units the zmpo_cunit array. */
+#define ZMPO_SEQ_SHAPED 0
+#define ZMPO_SEQ_STREAM 1
+#define ZMPO_SEQ_CHAIN 2
+#define ZMPO_SEQ_CHAIN16 3
+typedef struct zmpo_seq_op {
+  int32_t kind, shape, kt, upw, rb, atile, kmax, nmax;
+  const void* tasks;
+  const void* units;
+  int64_t nunits;
+  const void* groups;
+  int64_t ngroups;
+  const void* pairs;
+  const void* blocks;
+  int32_t slot_a, slot_b, slot_b2, slot_c, slot_work, beta, out_f32, pad;
+} zmpo_seq_op;
+typedef struct zmpo_seq zmpo_seq;
+int zmpo_seq_create(zmpo_seq** out);
+int zmpo_seq_destroy(zmpo_seq* s);
+int zmpo_seq_add(zmpo_seq* s, const zmpo_seq_op* op);
+int64_t zmpo_seq_length(const zmpo_seq* s);
+int zmpo_seq_run(const zmpo_seq* s, int dtype, void* const* slots, int nslots, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

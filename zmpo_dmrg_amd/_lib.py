@@ -43,12 +43,23 @@ SUNIT_DTYPE = np.dtype([('c_off', '<i8'), ('c_smhi', '<i8'), ('c_mlo', '<i4'), (
                         ('c_sn', '<i4'), ('m0', '<i4'), ('n0', '<i4'), ('M', '<i4'), ('N', '<i4'),
                         ('p_begin', '<i4'), ('p_end', '<i4'), ('slot', '<i4'), ('pad0', '<i4'),
                         ('pad1', '<i4')])
+SEQ_OP_DTYPE = np.dtype([('kind', '<i4'), ('shape', '<i4'), ('kt', '<i4'), ('upw', '<i4'),
+                         ('rb', '<i4'), ('atile', '<i4'), ('kmax', '<i4'), ('nmax', '<i4'),
+                         ('tasks', '<u8'), ('units', '<u8'), ('nunits', '<i8'), ('groups', '<u8'),
+                         ('ngroups', '<i8'), ('pairs', '<u8'), ('blocks', '<u8'),
+                         ('slot_a', '<i4'), ('slot_b', '<i4'), ('slot_b2', '<i4'),
+                         ('slot_c', '<i4'), ('slot_work', '<i4'), ('beta', '<i4'),
+                         ('out_f32', '<i4'), ('pad', '<i4')])
+SEQ_SHAPED, SEQ_STREAM, SEQ_CHAIN, SEQ_CHAIN16 = 0, 1, 2, 3
+H2D, D2H, D2D = 0, 1, 2
 TILE = 64
 
 EXPORTS = ['zmpo_last_error', 'zmpo_abi_version', 'zmpo_struct_sizes', 'zmpo_symAllowed',
            'zmpo_symAllowedInt', 'zmpo_blockgemm', 'zmpo_wavegemm', 'zmpo_shapedgemm', 'zmpo_streamgemm', 'zmpo_chaingemm', 'zmpo_chaingemm16', 'zmpo_skinnygemm', 'zmpo_dotgemm', 'zmpo_dots', 'zmpo_dots_workspace',
            'zmpo_lincomb', 'zmpo_precond', 'zmpo_gather', 'zmpo_scatter', 'zmpo_convert',
-           'zmpo_scratch_bytes', 'zmpo_streamgemm_static']
+           'zmpo_scratch_bytes', 'zmpo_streamgemm_static', 'zmpo_alloc', 'zmpo_free',
+           'zmpo_memcpy', 'zmpo_seq_create', 'zmpo_seq_destroy', 'zmpo_seq_add',
+           'zmpo_seq_length', 'zmpo_seq_run']
 
 
 class ZmpoError(RuntimeError):
@@ -90,6 +101,15 @@ def lib():
     L.zmpo_streamgemm_static.argtypes = [i32, i32, i32, vp, i64, i32, vp, vp, i64, vp, vp, vp,
                                          vp, i64, i32, vp]
     L.zmpo_scratch_bytes.restype = i64
+    L.zmpo_alloc.argtypes = [ctypes.POINTER(vp), i64]
+    L.zmpo_free.argtypes = [vp]
+    L.zmpo_memcpy.argtypes = [vp, vp, i64, i32, vp]
+    L.zmpo_seq_create.argtypes = [ctypes.POINTER(vp)]
+    L.zmpo_seq_destroy.argtypes = [vp]
+    L.zmpo_seq_add.argtypes = [vp, vp]
+    L.zmpo_seq_length.argtypes = [vp]
+    L.zmpo_seq_length.restype = i64
+    L.zmpo_seq_run.argtypes = [vp, i32, vp, i32, vp]
     L.zmpo_dots.argtypes = [i32, vp, i64, i32, vp, i64, vp, vp, vp]
     L.zmpo_dots_workspace.argtypes = [i32]
     L.zmpo_dots_workspace.restype = i32
@@ -130,3 +150,30 @@ def sym_allowed(nsyms, status, qsyms):
     ptrs = [q.ctypes.data for q in qs] + [None] * (8 - rank)
     check(lib().zmpo_symAllowed(rank, nq, nd.ctypes.data, st.ctypes.data, out.ctypes.data, *ptrs))
     return out
+
+
+class Seq:
+    """A launch sequence (zmpo_seq_*): ops recorded once, run with operand
+    slots bound to device pointers.  The recorded tables must outlive it."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().zmpo_seq_create(ctypes.byref(h)))
+        self._h = h
+        self._lib = lib()
+        self.nops = 0
+
+    def add(self, rec):
+        r = np.ascontiguousarray(np.asarray(rec, dtype=SEQ_OP_DTYPE).reshape(1))
+        check(self._lib.zmpo_seq_add(self._h, r.ctypes.data))
+        self.nops += 1
+
+    def run(self, dtype, slots, stream):
+        arr = (ctypes.c_void_p * len(slots))(*[int(x) if x else None for x in slots])
+        check(self._lib.zmpo_seq_run(self._h, dtype, arr, len(slots), stream))
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value:
+            self._lib.zmpo_seq_destroy(h)
+            self._h = None

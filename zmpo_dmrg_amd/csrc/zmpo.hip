@@ -16,6 +16,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/zmpo.h"
 
@@ -38,6 +39,7 @@ static_assert(sizeof(zmpo_task) == 56, "zmpo_task layout");
 static_assert(sizeof(zmpo_pair) == 64, "zmpo_pair layout");
 static_assert(sizeof(zmpo_unit) == 32, "zmpo_unit layout");
 static_assert(sizeof(zmpo_rgroup) == 24, "zmpo_rgroup layout");
+static_assert(sizeof(zmpo_seq_op) == 120, "zmpo_seq_op layout");
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
@@ -2161,6 +2163,113 @@ int zmpo_convert(int dtype_dst, void* dst, int dtype_src, const void* src, int64
   ZMPO_CONV(ZMPO_F32, float, ZMPO_BF16, bf16s)
 #undef ZMPO_CONV
   return set_error(ZMPO_ERR_ARG, "convert: unsupported dtype pair");
+}
+
+// ------------------------------------------------------------ device store
+int zmpo_alloc(void** out, int64_t bytes) {
+  if (!out || bytes < 0) return set_error(ZMPO_ERR_ARG, "alloc: bad arguments");
+  *out = nullptr;
+  if (bytes == 0) return ZMPO_OK;
+  return check_hip(hipMalloc(out, (size_t)bytes), "alloc");
+}
+
+int zmpo_free(void* p) {
+  if (!p) return ZMPO_OK;
+  return check_hip(hipFree(p), "free");
+}
+
+int zmpo_memcpy(void* dst, const void* src, int64_t bytes, int kind, void* stream) {
+  if (bytes <= 0) return ZMPO_OK;
+  if (!dst || !src) return set_error(ZMPO_ERR_ARG, "memcpy: null pointer");
+  hipMemcpyKind k;
+  switch (kind) {
+    case ZMPO_H2D: k = hipMemcpyHostToDevice; break;
+    case ZMPO_D2H: k = hipMemcpyDeviceToHost; break;
+    case ZMPO_D2D: k = hipMemcpyDeviceToDevice; break;
+    default: return set_error(ZMPO_ERR_ARG, "memcpy: unknown direction");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int rc = check_hip(hipMemcpyAsync(dst, src, (size_t)bytes, k, s), "memcpy");
+  if (rc || kind == ZMPO_D2D) return rc;
+  // host buffers are pageable: the copy is complete when the call returns
+  return check_hip(hipStreamSynchronize(s), "memcpy sync");
+}
+
+// ---------------------------------------------------------- launch sequences
+// A recorded list of grouped-GEMM and chained launches over numbered operand
+// slots: a whole sigma (or renormalisation, or diagonal) is one call that
+// binds the slots to device buffers and issues every launch in order on one
+// stream.  Tables are device pointers owned by the caller.
+struct zmpo_seq {
+  std::vector<zmpo_seq_op> ops;
+};
+
+}  // extern "C"
+
+namespace {
+int seq_check_slot(int x, int nslots) { return x < nslots; }
+}  // namespace
+
+extern "C" {
+
+int zmpo_seq_create(zmpo_seq** out) {
+  if (!out) return set_error(ZMPO_ERR_ARG, "seq_create: null pointer");
+  *out = new (std::nothrow) zmpo_seq();
+  return *out ? ZMPO_OK : set_error(ZMPO_ERR_ARG, "seq_create: out of host memory");
+}
+
+int zmpo_seq_destroy(zmpo_seq* s) {
+  delete s;
+  return ZMPO_OK;
+}
+
+int zmpo_seq_add(zmpo_seq* s, const zmpo_seq_op* op) {
+  if (!s || !op) return set_error(ZMPO_ERR_ARG, "seq_add: null pointer");
+  if (op->kind < ZMPO_SEQ_SHAPED || op->kind > ZMPO_SEQ_CHAIN16)
+    return set_error(ZMPO_ERR_ARG, "seq_add: unknown launch kind");
+  if (op->slot_a < 0 || op->slot_b < 0 || op->slot_c < 0 ||
+      ((op->kind == ZMPO_SEQ_CHAIN || op->kind == ZMPO_SEQ_CHAIN16) && op->slot_b2 < 0))
+    return set_error(ZMPO_ERR_ARG, "seq_add: missing operand slot");
+  s->ops.push_back(*op);
+  return ZMPO_OK;
+}
+
+int64_t zmpo_seq_length(const zmpo_seq* s) { return s ? (int64_t)s->ops.size() : -1; }
+
+int zmpo_seq_run(const zmpo_seq* s, int dtype, void* const* slots, int nslots, void* stream) {
+  if (!s || (nslots > 0 && !slots)) return set_error(ZMPO_ERR_ARG, "seq_run: null pointer");
+  for (const zmpo_seq_op& op : s->ops) {
+    const int hi = op.slot_a > op.slot_b ? op.slot_a : op.slot_b;
+    const int hi2 = op.slot_c > op.slot_b2 ? op.slot_c : op.slot_b2;
+    if (!seq_check_slot(hi > hi2 ? hi : hi2, nslots) || !seq_check_slot(op.slot_work, nslots))
+      return set_error(ZMPO_ERR_ARG, "seq_run: operand slot beyond nslots");
+    void* A = slots[op.slot_a];
+    void* B = slots[op.slot_b];
+    void* C = slots[op.slot_c];
+    void* W = op.slot_work >= 0 ? slots[op.slot_work] : nullptr;
+    const int dt = (op.out_f32 && dtype == ZMPO_BF16) ? ZMPO_BF16_F32 : dtype;
+    int rc = ZMPO_OK;
+    switch (op.kind) {
+      case ZMPO_SEQ_SHAPED:
+        rc = zmpo_shapedgemm(dt, op.shape, (const zmpo_task*)op.tasks, (const zmpo_unit*)op.units,
+                             op.nunits, (const zmpo_rgroup*)op.groups, op.ngroups,
+                             (const zmpo_pair*)op.pairs, A, B, C, W, op.beta, stream);
+        break;
+      case ZMPO_SEQ_STREAM:
+        rc = zmpo_streamgemm(dt, op.shape, op.kt, (const zmpo_sunit*)op.units, op.nunits, op.upw,
+                             (const zmpo_task*)op.tasks, (const zmpo_rgroup*)op.groups, op.ngroups,
+                             (const zmpo_pair*)op.pairs, A, B, C, W, op.beta, stream);
+        break;
+      default:
+        rc = (op.kind == ZMPO_SEQ_CHAIN ? zmpo_chaingemm : zmpo_chaingemm16)(
+            dt, op.rb, op.atile, op.kmax, op.nmax, (const zmpo_cunit*)op.units, op.nunits,
+            (const zmpo_cgroup*)op.groups, (const zmpo_cblock*)op.blocks,
+            (const zmpo_task*)op.tasks, (const zmpo_pair*)op.pairs, A, B, slots[op.slot_b2], C,
+            op.beta, stream);
+    }
+    if (rc) return rc;
+  }
+  return ZMPO_OK;
 }
 
 }  // extern "C"

@@ -110,8 +110,12 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
         t0 = lap('diag', t0)
         prog = kernelQt._sigma_program(dmrg, isite, fL, fR)
         t0 = lap('sigma_plan', t0)
+        dmrg.last_step = {'ndim0': ndim0, 'sigma_gflop': prog.flops / 1e9,
+                          'sectors': (len(dmrg.qtmp.ndims[0]), len(dmrg.qtmp.ndims[3])),
+                          'dims': (int(dmrg.qtmp.shape[0]), int(dmrg.qtmp.shape[3]))}
         dtype = getattr(dmrg, 'dtype', 0)
         solver = EigenSolver()
+        solver.maxcycle = getattr(dmrg, 'maxcycle', solver.maxcycle)
         solver.crit_e = dmrg.crit_e
         solver.crit_vec = dmrg.crit_vec
         solver.ndim = ndim0

@@ -28,7 +28,8 @@ import numpy as np
 
 from . import _lib
 from . import device as dv
-from .qtensor import ChainProgram, Program, QTensor, chain_structure, gather, run_chain, run_program
+from .qtensor import (ChainProgram, Program, QTensor, _workspace, chain_structure, gather,
+                      run_chain, run_program, seq_add_chain, seq_add_program)
 
 ENV_PERM = (1, 0, 2)
 TOP1_PERM = (1, 0, 2, 3, 4)
@@ -56,6 +57,11 @@ STAGE4_KT.update({2: 32, 3: 32})
 # (measured at K = 36, D = 1000: L x psi 15.6 ms vs 16.6 for 64x16, x R 13.9
 # ms vs 17.5 for zmpo_dotgemm)
 SIGMA_SHAPES = [os.environ.get('ZMPO_SHAPE_S1', '2'), os.environ.get('ZMPO_SHAPE_S4', '2')]
+# issue a whole sigma through one recorded launch sequence (zmpo_seq_run)
+# instead of one library call per launch
+SIGMA_SEQ = os.environ.get('ZMPO_SEQ', '1') != '0'
+# operand slots of a sigma sequence
+S_L, S_R, S_PI, S_PJ, S_PSI, S_SIG, S_T1, S_T3, S_WORK = range(9)
 # ZMPO_PLAN_CACHE=1: keep each site's sigma program (plan tables only; the
 # intermediates are released between steps) and reuse it when a later sweep
 # meets the same block structure at that site
@@ -305,6 +311,33 @@ class SigmaProgram:
         self.flops = int(self.flops_stage.sum())
         self._buffers = {}
 
+    def sequence(self, dtype):
+        """The recorded launch sequence of a fused sigma in dtype (built once
+        per dtype; None when a launch cannot be recorded).  Returns (seq,
+        workspace elements)."""
+        seqs = self.__dict__.setdefault('_seqs', {})
+        if dtype not in seqs:
+            seq = _lib.Seq()
+            work = 0
+            first = True
+            ok = self.fused
+            for st, chain in zip(self.stages, self.chains):
+                if not ok:
+                    break
+                w = seq_add_program(seq, st[0], dtype, S_L, S_PSI, S_T1, S_WORK)
+                seq_add_chain(seq, chain, S_T1, S_PI, S_PJ, S_T3)
+                w4 = 0
+                if st[3].ntasks:
+                    w4 = seq_add_program(seq, st[3], out_dtype(dtype), S_T3, S_R, S_SIG, S_WORK,
+                                         beta=0 if first else 1, out_f32=True)
+                    first = False
+                if w is None or w4 is None:
+                    ok = False
+                else:
+                    work = max(work, w, w4)
+            seqs[dtype] = (seq, work) if ok and not first else None
+        return seqs[dtype]
+
     def device_refs(self):
         """The uploaded plan tables and schedules the launches read (a
         captured graph keeps them alive)."""
@@ -322,7 +355,8 @@ class SigmaProgram:
 
     def release_device(self):
         """Drop the uploaded plan tables of every launch (host arrays stay;
-        the next apply uploads them again)."""
+        the next apply uploads them again) and the sequences recording them."""
+        self.__dict__.pop('_seqs', None)
         for st in self.stages:
             for pg in st:
                 pg.release_device()
@@ -360,6 +394,18 @@ class SigmaProgram:
         if fused and overlap and len(self.stages) > 1 and not isinstance(psi, np.ndarray) \
                 and getattr(psi, 'is_cuda', False):
             return self._apply_overlapped(mpo, L, R, psi, sigma, dtype, t1, t3)
+        if fused and SIGMA_SEQ and getattr(psi, 'is_cuda', False):
+            sq = self.sequence(dtype)
+            if sq is not None:
+                seq, nwork = sq
+                work = _workspace(nwork, out_dtype(dtype)) if nwork else None
+                # the sigma's split-K slots and L x psi's share one workspace;
+                # they are FP32 in BF16 runs (L x psi never splits at these K)
+                slots = [dv.ptr(L), dv.ptr(R), dv.ptr(mpo.padded_buffer(self.isite, dtype)),
+                         dv.ptr(mpo.padded_buffer(self.isite + 1, dtype)), dv.ptr(psi),
+                         dv.ptr(sigma), dv.ptr(t1), dv.ptr(t3), dv.ptr(work)]
+                seq.run(dtype, slots, dv.stream())
+                return sigma
         first = True
         for st, chain in zip(self.stages, self.chains):
             run_program(st[0], L, psi, t1, dtype)

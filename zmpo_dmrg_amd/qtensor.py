@@ -872,6 +872,63 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None)
                                      dv.ptr(work), beta, st))
 
 
+def program_kernel(prog, kernel=None):
+    """The grouped-GEMM kernel run_program would use for prog."""
+    if kernel is None:
+        kernel = prog.prefer if prog.prefer else KERNEL
+    if kernel == 'stream' and not prog.small_blocks:
+        kernel = 'shaped'
+    if kernel == 'skinny' and not prog.skinny_ok():
+        kernel = 'shaped'
+    return kernel
+
+
+def seq_add_program(seq, prog, dtype, a, b, c, work, beta=0, out_f32=False):
+    """Record prog (as run_program would launch it in dtype) into seq with
+    operand slots a, b, c and split-K workspace slot work.  Returns the
+    workspace elements it needs, or None if its kernel cannot be recorded
+    (only the shaped and stream kernels can)."""
+    if prog.ntasks == 0:
+        return 0
+    kernel = program_kernel(prog)
+    if kernel not in ('shaped', 'stream'):
+        return None
+    tasks, pairs = prog.device()
+    sc = prog.schedule(kernel)
+    rec = np.zeros(1, _lib.SEQ_OP_DTYPE)
+    rec['kind'] = _lib.SEQ_STREAM if kernel == 'stream' else _lib.SEQ_SHAPED
+    rec['shape'] = prog.shape
+    if kernel == 'stream':
+        kt = prog.kt_dtype.get(dtype, sc['kt']) if prog.kt_dtype else sc['kt']
+        rec['kt'], rec['upw'] = kt, sc['upw']
+    rec['tasks'], rec['pairs'] = dv.ptr(tasks), dv.ptr(pairs)
+    rec['units'], rec['nunits'] = dv.ptr(sc['units']), sc['nunits']
+    rec['groups'], rec['ngroups'] = dv.ptr(sc['groups']), sc['ngroups']
+    rec['slot_a'], rec['slot_b'], rec['slot_b2'], rec['slot_c'] = a, b, -1, c
+    rec['slot_work'] = work if sc.get('nslots') else -1
+    rec['beta'], rec['out_f32'] = beta, int(bool(out_f32))
+    seq.add(rec)
+    return sc['nslots'] * sc['slot_size'] if sc.get('nslots') else 0
+
+
+def seq_add_chain(seq, prog, a, b1, b2, c, beta=0):
+    """Record every launch of a ChainProgram (run_chain) into seq."""
+    if prog.nunits == 0:
+        return
+    (g, bl, t, p), units = prog.device()
+    for l_, u in zip(prog.launches, units):
+        rec = np.zeros(1, _lib.SEQ_OP_DTYPE)
+        rec['kind'] = _lib.SEQ_CHAIN16 if l_.get('n16') else _lib.SEQ_CHAIN
+        rec['rb'], rec['atile'], rec['kmax'], rec['nmax'] = prog.rb, l_['atile'], l_['kmax'], l_['nmax']
+        rec['tasks'], rec['pairs'] = dv.ptr(t), dv.ptr(p)
+        rec['units'], rec['nunits'] = dv.ptr(u), len(l_['units'])
+        rec['groups'], rec['blocks'] = dv.ptr(g), dv.ptr(bl)
+        rec['slot_a'], rec['slot_b'], rec['slot_b2'], rec['slot_c'] = a, b1, b2, c
+        rec['slot_work'] = -1
+        rec['beta'] = beta
+        seq.add(rec)
+
+
 def _map_lookup(offs, bmap):
     """(new offset, row stride) of every B block offset through a padded-copy map."""
     if len(offs) == 0:

@@ -65,8 +65,11 @@ def main(argv=None):
         return f() / 2 ** 30
 
     def step_log(status, isite, eig, dt, nsig):
-        print('[run]   %s site %2d E=%.10f %.2fs sigma %d mem %.1f GiB' % (
-            status, isite, eig, dt, nsig, gib()), flush=True)
+        ls = getattr(d, 'last_step', {})
+        print('[run]   %s site %2d E=%.10f %.2fs sigma %d mem %.1f GiB ndim0 %s sigma %.1f GFLOP '
+              'sectors %s dims %s' % (status, isite, eig, dt, nsig, gib(), ls.get('ndim0'),
+                                      ls.get('sigma_gflop', 0.0), ls.get('sectors'),
+                                      ls.get('dims')), flush=True)
 
     d.step_log = step_log
     out = {'energies': [], 'sweep_s': [], 'Dmax': [], 'nsigma': []}
