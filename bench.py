@@ -28,7 +28,15 @@ def log(msg):
     print('[bench] ' + msg, file=sys.stderr, flush=True)
 
 
-def build_workload(K, D, seed, dtype, budget, fused=True, extra=None):
+# (N, Sz) sector widths of the synthetic MPS.  'dmrg' matches the middle
+# bond of a real K = 36, D = 1000 DMRG state (the first D = 1000 sweep of a
+# Normal2 ramp from the product state: 24 / 21 sectors, ndim0 1.20e6, 5.7
+# TFLOP per sigma; synthetic: 28 / 28 sectors, ndim0 1.18e6); 'wide' is the
+# generator's default, with many small sectors (116 at K = 36, ndim0 2.8e5).
+STRUCTURES = {'dmrg': lambda K: (0.5 + 0.02 * K, 0.35 + 0.01 * K), 'wide': lambda K: (None, None)}
+
+
+def build_workload(K, D, seed, dtype, budget, fused=True, extra=None, structure='dmrg'):
     """Middle-bond sigma workload: (mpo, isite, st, L, R, prog).  ``extra``
     (a dict) also receives the environments one site before L and R with
     the site tensors that renormalise them into L and R."""
@@ -43,7 +51,9 @@ def build_workload(K, D, seed, dtype, budget, fused=True, extra=None):
     mpo = pmpo.MPOStore(i1, i2)
     log('MPO K=%d built in %.1fs' % (K, time.time() - t0))
     isite = K // 2 - 1
-    ls, rs, ql, qr = synthetic.synthetic_mps(K, K, D, isite - 1, isite + 2, seed=seed)
+    sN, sS = STRUCTURES[structure](K)
+    ls, rs, ql, qr = synthetic.synthetic_mps(K, K, D, isite - 1, isite + 2, seed=seed, sN=sN,
+                                             sS=sS)
     if extra is not None:
         extra.update(ls=ls, rs=rs, ql=ql, qr=qr)
     log('synthetic MPS: bond dims L %s R %s' % (len(ql[isite]), len(qr[isite + 2])))
@@ -266,7 +276,9 @@ def main():
     ap.add_argument('--dtype', choices=['f64', 'f32'], default='f64')
     ap.add_argument('--budget-gb', type=float, default=96.0)
     ap.add_argument('--cpu-ops', type=int, default=2)
-    ap.add_argument('--window', type=int, default=2,
+    ap.add_argument('--structure', choices=sorted(STRUCTURES), default='dmrg',
+                    help='sector structure of the synthetic MPS')
+    ap.add_argument('--window', type=int, default=1,
                     help='full two-site steps timed per precision after the sigma (0: none)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-f32', action='store_true',
@@ -290,7 +302,8 @@ def main():
     extra = {}
     mpo, isite, st, L, R, prog = build_workload(args.K, args.D, args.seed, dtype,
                                                 int(args.budget_gb * 2 ** 30),
-                                                fused=not args.unfused, extra=extra)
+                                                fused=not args.unfused, extra=extra,
+                                                structure=args.structure)
     rng = np.random.default_rng(1)
     psi_h = rng.standard_normal(st.qtmp.size_allowed)
     psi_h /= np.linalg.norm(psi_h)
@@ -420,7 +433,8 @@ def main():
         'value': gflops, 'unit': 'GFLOP/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': ms_step, 'higher_is_better': True,
         'scaling': 'weak', 'vs_baseline': None, 'dtype': args.dtype,
-        'data': 'synthetic (seeded integrals; synthetic block-sparse MPS, device-built envs)',
+        'data': 'synthetic (seeded integrals; synthetic block-sparse MPS with the sector '
+                'structure of a real K=36 D=1000 DMRG middle bond (%s), device-built envs)' % args.structure,
         'config': {'workload': 'middle-bond two-site sigma, K=%d N=%d Sz=0 D=%d, nops=%d' % (
             args.K, args.K, args.D, mpo.nops), 'ndim0': int(st.qtmp.size_allowed),
             'sigma_gflop': prog.flops / 1e9, 'chunks': len(prog.chunks),

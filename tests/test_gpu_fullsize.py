@@ -28,7 +28,7 @@ def full(gpu):
     import bench
     print('[fullsize] building K=36 D=1000', flush=True)
     mpo, isite, st, L, R, prog = bench.build_workload(36, 1000, 0, 0, 96 * 2 ** 30, fused=True,
-                                                      extra=EXTRA)
+                                                      extra=EXTRA, structure='wide')
     print('[fullsize] K=36 D=1000 ndim0=%d' % st.qtmp.size_allowed, flush=True)
     yield mpo, isite, st, L, R, prog
     EXTRA.clear()

@@ -15,7 +15,8 @@ EXTRA = {}
 def k24(gpu):
     import bench
     print('[k24] building K=24 D=500', flush=True)
-    w = bench.build_workload(24, 500, 1, 0, 32 * 2 ** 30, fused=True, extra=EXTRA)
+    w = bench.build_workload(24, 500, 1, 0, 32 * 2 ** 30, fused=True, extra=EXTRA,
+                             structure='wide')
     yield w
     EXTRA.clear()
 

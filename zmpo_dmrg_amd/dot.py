@@ -237,7 +237,10 @@ def _defect_correction(dmrg, solver, hvec_lo, hvec64, v0, n, dtype):
     returns x_(k+1).  The operator error of the inner solve enters only
     through the correction x_(k+1) - x_k, so each outer step shrinks the
     error by about |H_lo - H| / gap; the reported eigenvalue is the FP64
-    Rayleigh quotient of the returned vector.  Returns (eigs, vec, nmvp)."""
+    Rayleigh quotient of the returned vector.  When an outer step fails to
+    halve the FP64 residual (the low-precision operator is too far from H
+    for this gap), the solve continues as a plain FP64 Davidson from the
+    current vector.  Returns (eigs, vec, nmvp)."""
     from .pdvdson import _VecOps
     t = dv.torch()
     ops = _VecOps(n)
@@ -251,7 +254,7 @@ def _defect_correction(dmrg, solver, hvec_lo, hvec64, v0, n, dtype):
     y = t.empty_like(x)
     r = t.empty_like(x)
     nmvp = 0
-    theta_prev = None
+    theta_prev = rn_prev = None
     herm = 1e-5 if dtype == 1 else 2e-2
     niter = 0
     for k in range(OUTER_MAX):
@@ -264,7 +267,17 @@ def _defect_correction(dmrg, solver, hvec_lo, hvec64, v0, n, dtype):
         if rn <= dmrg.crit_vec or (theta_prev is not None
                                    and abs(theta - theta_prev) <= dmrg.crit_e):
             break
-        theta_prev = theta
+        if rn_prev is not None and rn > 0.5 * rn_prev:
+            fine = EigenSolver()
+            fine.maxcycle = getattr(dmrg, 'maxcycle', fine.maxcycle)
+            fine.crit_e, fine.crit_vec = dmrg.crit_e, dmrg.crit_vec
+            fine.ndim, fine.neig, fine.diag = n, 1, solver.diag
+            fine.HVec = hvec64
+            eigs, vec, nm = fine.solve_iter(x.view(1, -1), w0=y.view(1, -1))
+            solver.niter = niter + fine.niter
+            dmrg.nfallback = getattr(dmrg, 'nfallback', 0) + 1
+            return eigs, vec, nmvp + nm
+        theta_prev, rn_prev = theta, rn
         inner = EigenSolver()
         inner.crit_vec = max(dmrg.crit_vec, INNER_FRAC * rn)
         inner.crit_e = -1.0                   # residual criterion only

@@ -718,6 +718,24 @@ class RenormProgram:
             self.stages.append(progs)
             self.inter = [max(a, b) for a, b in zip(self.inter, offs)]
 
+    def sequence(self, dtype):
+        """The three launches per op chunk as one recorded sequence over the
+        slots (site, environment, MPO, t1, t2, out, workspace); None when a
+        launch cannot be recorded."""
+        seqs = self.__dict__.setdefault('_seqs', {})
+        if dtype not in seqs:
+            seq = _lib.Seq()
+            work, ok = 0, True
+            for st in self.stages:
+                for pg, a, b, c in ((st[0], 0, 1, 3), (st[1], 2, 3, 4), (st[2], 4, 0, 5)):
+                    w = seq_add_program(seq, pg, dtype, a, b, c, 6)
+                    if w is None:
+                        ok = False
+                    else:
+                        work = max(work, w)
+            seqs[dtype] = (seq, work) if ok else None
+        return seqs[dtype]
+
     def apply(self, mpo, site_values, env_in, dtype=0):
         """Return the new EnvStore (values on the device)."""
         t1 = dv.empty(self.inter[0], dtype)
@@ -725,10 +743,17 @@ class RenormProgram:
         out = dv.zeros(self.out_size, dtype)
         W = mpo.buffer(self.site_idx, dtype)
         E = env_in.buffer(dtype)
-        for st in self.stages:
-            run_program(st[0], site_values, E, t1, dtype)
-            run_program(st[1], W, t1, t2, dtype)
-            run_program(st[2], t2, site_values, out, dtype)
+        sq = self.sequence(dtype) if SIGMA_SEQ and getattr(out, 'is_cuda', False) else None
+        if sq is not None:
+            seq, nwork = sq
+            work = _workspace(nwork, dtype) if nwork else None
+            seq.run(dtype, [dv.ptr(x) for x in (site_values, E, W, t1, t2, out, work)],
+                    dv.stream())
+        else:
+            for st in self.stages:
+                run_program(st[0], site_values, E, t1, dtype)
+                run_program(st[1], W, t1, t2, dtype)
+                run_program(st[2], t2, site_values, out, dtype)
         if dtype != 0:
             o64 = dv.empty(self.out_size, 0)
             _lib.check(_lib.lib().zmpo_convert(0, dv.ptr(o64), dtype, dv.ptr(out),
