@@ -24,7 +24,7 @@ import numpy as np
 
 from . import device as dv
 from . import kernelQt
-from .pdvdson import EigenSolver
+from .pdvdson import DavidsonError, EigenSolver
 from .qtensor import QTensor, fromQnums
 
 
@@ -257,6 +257,17 @@ def _defect_correction(dmrg, solver, hvec_lo, hvec64, v0, n, dtype):
     theta_prev = rn_prev = None
     herm = 1e-5 if dtype == 1 else 2e-2
     niter = 0
+
+    def fallback():
+        fine = EigenSolver()
+        fine.maxcycle = getattr(dmrg, 'maxcycle', fine.maxcycle)
+        fine.crit_e, fine.crit_vec = dmrg.crit_e, dmrg.crit_vec
+        fine.ndim, fine.neig, fine.diag = n, 1, solver.diag
+        fine.HVec = hvec64
+        eigs, vec, nm = fine.solve_iter(x.view(1, -1), w0=y.view(1, -1))
+        solver.niter = niter + fine.niter
+        dmrg.nfallback = getattr(dmrg, 'nfallback', 0) + 1
+        return eigs, vec, nmvp + nm
     for k in range(OUTER_MAX):
         hvec64(x, y)
         nmvp += 1
@@ -268,15 +279,7 @@ def _defect_correction(dmrg, solver, hvec_lo, hvec64, v0, n, dtype):
                                    and abs(theta - theta_prev) <= dmrg.crit_e):
             break
         if rn_prev is not None and rn > 0.5 * rn_prev:
-            fine = EigenSolver()
-            fine.maxcycle = getattr(dmrg, 'maxcycle', fine.maxcycle)
-            fine.crit_e, fine.crit_vec = dmrg.crit_e, dmrg.crit_vec
-            fine.ndim, fine.neig, fine.diag = n, 1, solver.diag
-            fine.HVec = hvec64
-            eigs, vec, nm = fine.solve_iter(x.view(1, -1), w0=y.view(1, -1))
-            solver.niter = niter + fine.niter
-            dmrg.nfallback = getattr(dmrg, 'nfallback', 0) + 1
-            return eigs, vec, nmvp + nm
+            return fallback()
         theta_prev, rn_prev = theta, rn
         inner = EigenSolver()
         inner.crit_vec = max(dmrg.crit_vec, INNER_FRAC * rn)
@@ -285,7 +288,13 @@ def _defect_correction(dmrg, solver, hvec_lo, hvec64, v0, n, dtype):
         inner.herm_tol = herm
         inner.ndim, inner.neig, inner.diag = n, 1, solver.diag
         inner.HVec = hvec_lo
-        _, vec, nm = inner.solve_iter(x.view(1, -1), w0=y.view(1, -1))
+        try:
+            _, vec, nm = inner.solve_iter(x.view(1, -1), w0=y.view(1, -1))
+        except DavidsonError:
+            # the low-precision search space ran out of independent
+            # directions (small H_eff, or rounding at the level of the
+            # residual): continue in FP64 from x_k
+            return fallback()
         nmvp += nm
         niter += inner.niter
         x.copy_(vec[0])
