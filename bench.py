@@ -28,73 +28,12 @@ def log(msg):
     print('[bench] ' + msg, file=sys.stderr, flush=True)
 
 
-# (N, Sz) sector widths of the synthetic MPS.  'dmrg' matches the middle
-# bond of a real K = 36, D = 1000 DMRG state (the first D = 1000 sweep of a
-# Normal2 ramp from the product state: 24 / 21 sectors, ndim0 1.20e6, 5.7
-# TFLOP per sigma; synthetic: 28 / 28 sectors, ndim0 1.18e6); 'wide' is the
-# generator's default, with many small sectors (116 at K = 36, ndim0 2.8e5).
-STRUCTURES = {'dmrg': lambda K: (0.5 + 0.02 * K, 0.35 + 0.01 * K), 'wide': lambda K: (None, None)}
-
-
 def build_workload(K, D, seed, dtype, budget, fused=True, extra=None, structure='dmrg'):
-    """Middle-bond sigma workload: (mpo, isite, st, L, R, prog).  ``extra``
-    (a dict) also receives the environments one site before L and R with
-    the site tensors that renormalise them into L and R."""
-    from zmpo_dmrg_amd import device as dv
-    from zmpo_dmrg_amd import kernelQt as kq
-    from zmpo_dmrg_amd import mpo as pmpo
+    """Middle-bond sigma workload: (mpo, isite, st, L, R, prog)
+    (zmpo_dmrg_amd.synthetic.middle_bond)."""
     from zmpo_dmrg_amd import synthetic
-    from zmpo_dmrg_amd.dot import setupQtmpCI
-    from zmpo_dmrg_amd.integrals import gen_integrals
-    t0 = time.time()
-    i1, i2 = gen_integrals(K, seed=seed)
-    mpo = pmpo.MPOStore(i1, i2)
-    log('MPO K=%d built in %.1fs' % (K, time.time() - t0))
-    isite = K // 2 - 1
-    sN, sS = STRUCTURES[structure](K)
-    ls, rs, ql, qr = synthetic.synthetic_mps(K, K, D, isite - 1, isite + 2, seed=seed, sN=sN,
-                                             sS=sS)
-    if extra is not None:
-        extra.update(ls=ls, rs=rs, ql=ql, qr=qr)
-    log('synthetic MPS: bond dims L %s R %s' % (len(ql[isite]), len(qr[isite + 2])))
-    env = kq.boundary_store(mpo.nops)
-    ren_flops = 0
-    ren_time = 0.0
-    for i in range(isite):
-        dv.sync()
-        t = time.time()
-        if extra is not None and i == isite - 1:
-            extra.update(Lprev=env, lsite=ls[i], lidx=i)
-        env, prog = kq.renorm(mpo, i, ls[i], env, 'L')
-        dv.sync()
-        ren_time += time.time() - t
-        ren_flops += prog.flops
-    L = env
-    env = kq.boundary_store(mpo.nops)
-    for i in range(K - 1, isite + 1, -1):
-        dv.sync()
-        t = time.time()
-        if extra is not None and i == isite + 2:
-            extra.update(Rprev=env, rsite=rs[i], ridx=i)
-        env, prog = kq.renorm(mpo, i, rs[i], env, 'R')
-        dv.sync()
-        ren_time += time.time() - t
-        ren_flops += prog.flops
-    R = env
-    log('environments built (%d renormalisations, %.1f GFLOP, %.1fs incl. plans)' % (
-        K - 2, ren_flops / 1e9, ren_time))
-
-    class St:
-        pass
-    st = St()
-    st.qlst = [ql[isite], pmpo.qphys(), pmpo.qphys(), qr[isite + 2]]
-    setupQtmpCI(st, (float(K), 0.0))
-    t = time.time()
-    prog = kq.SigmaProgram(mpo, isite, st.qtmp, st.qkey, L, R, mpo.hpwts, budget=budget,
-                           fused=fused)
-    log('sigma plan: ndim0=%d flops=%.3e chunks=%d (%.1fs)' % (
-        st.qtmp.size_allowed, prog.flops, len(prog.chunks), time.time() - t))
-    return mpo, isite, st, L, R, prog
+    return synthetic.middle_bond(K, D, seed, budget, fused=fused, extra=extra,
+                                 structure=structure, log=log)
 
 
 def time_stages(prog, mpo, L, R, psi, sig, dtype, reps=3):
@@ -205,8 +144,12 @@ def step_window(mpo, isite, L, R, extra, D, dtype, budget, nsteps=2, tol=1e-8, m
         times.append(time.time() - t)
         eigs.append(float(e[0]))
         log('window dtype %d step %d: E=%.10f %.2fs' % (dtype, i, eigs[-1], times[-1]))
+    tiers = {str(k): v for k, v in getattr(d, 'nsigma_tier', {}).items()}
+    if dtype:
+        tiers[str(dtype)] = d.nsigma - sum(getattr(d, 'nsigma_tier', {}).values())
     return {'step_s': times, 'eigs': eigs, 'nsigma_low': d.nsigma if dtype else 0,
             'nsigma_f64': getattr(d, 'nsigma64', 0) if dtype else d.nsigma,
+            'nsigma_by_dtype': tiers, 'fallbacks': getattr(d, 'nfallback', 0),
             'timing': {k: round(v, 3) for k, v in d.timing.items()}}
 
 
@@ -276,9 +219,9 @@ def main():
     ap.add_argument('--dtype', choices=['f64', 'f32'], default='f64')
     ap.add_argument('--budget-gb', type=float, default=96.0)
     ap.add_argument('--cpu-ops', type=int, default=2)
-    ap.add_argument('--structure', choices=sorted(STRUCTURES), default='dmrg',
+    ap.add_argument('--structure', choices=['dmrg', 'wide'], default='dmrg',
                     help='sector structure of the synthetic MPS')
-    ap.add_argument('--window', type=int, default=1,
+    ap.add_argument('--window', type=int, default=2,
                     help='full two-site steps timed per precision after the sigma (0: none)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-f32', action='store_true',
@@ -407,12 +350,19 @@ def main():
             w['dE_first_step'] = abs(w['eigs'][0] - exact)
             w['step_speedup'] = sum(base['step_s']) / sum(w['step_s'])
         nstep = 2 * (args.K - 1)
+        # steps after the first start from the reference's guess, as sweep
+        # steps do; the first starts from the lowest diagonal element
+        warm = lambda w: w['step_s'][1:] if len(w['step_s']) > 1 else w['step_s']
         window = {'steps': args.window, 'tol': 1e-8, 'D': args.D, 'first_step_exact': exact,
-                  'sweep_s_estimate': {str(dt): round(nstep * float(np.mean(window[dt]['step_s'])), 1)
+                  'sweep_s_estimate': {str(dt): round(nstep * float(np.mean(warm(window[dt]))), 1)
                                        for dt in window},
-                  'how': 'mean of the window steps (middle bond, the largest blocks) x 2(K-1) '
-                         'steps per sweep; dtype 0 FP64, 1 FP32 + FP64 correction, 2 BF16 + '
-                         'FP64 correction',
+                  'speedup_vs_f64': {str(dt): round(float(np.mean(warm(window[0])))
+                                                    / float(np.mean(warm(window[dt]))), 3)
+                                     for dt in window},
+                  'how': 'mean of the window steps that start from the guess (middle bond, '
+                         'the largest blocks) x 2(K-1) steps per sweep, an upper estimate; '
+                         'dtype 0 FP64, 1 FP32 + FP64 correction, 2 BF16 then FP32 + FP64 '
+                         'correction',
                   'runs': {str(k): v for k, v in window.items()}}
         log('window: %s' % json.dumps(window['sweep_s_estimate']))
     cpu = None

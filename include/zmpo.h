@@ -145,6 +145,17 @@ int zmpo_wavegemm(int dtype, int rb, const zmpo_task* tasks, const zmpo_unit* un
 #define ZMPO_TILE_64x32 3
 #define ZMPO_TILE_16x16 4
 #define ZMPO_TILE_64x64 5
+/* Register-blocked panel tiles (same units, split-K groups and results up
+ * to the order of the k-sum inside a k-tile): the 4 waves of a workgroup
+ * form a 2x2 grid of (TM/2)x(TN/2) register blocks; operand tiles sit in
+ * LDS with k contiguous (16 k per tile for FP64/FP32, 32 for BF16, which
+ * runs on the gfx950 16x16x32 BF16 MFMA).  For blocks of ~100 and more. */
+#define ZMPO_PANEL_64x64 6
+#define ZMPO_PANEL_128x64 7
+#define ZMPO_PANEL_64x128 8
+#define ZMPO_PANEL_128x128 9
+#define ZMPO_PANEL_32x64 10
+#define ZMPO_PANEL_64x32 11
 
 /* LDS-staged grouped GEMM with block-shaped tiles: one workgroup per
  * zmpo_unit (unit m0/n0 step by the tile shape), same split-K groups and

@@ -438,6 +438,9 @@ struct TileLoader {
   int64_t ks;
   int K;
   M scale;
+  T v[NE];       // fetched values (committed to LDS later)
+  M vscale;      // scale of the pair the values belong to
+  unsigned vok;  // which of v are real elements
   __device__ __forceinline__ void setup(int tid, const T* b, int64_t off, int K_, int64_t sk, int mlo,
                         int64_t shi, int64_t slo, int r0, int rmax, M sc) {
     base = b + off;
@@ -462,13 +465,24 @@ struct TileLoader {
       roff[i] = r < rmax ? o : (int64_t)-1;
     }
   }
-  __device__ __forceinline__ void load(T (*dst)[ROWS + 1], int k0) const {
+  // Loads of one k-tile into registers, issued unconditionally (padding
+  // elements read the block's first element) so that they stay in flight
+  // until commit(), which zeroes padding, scales and writes the LDS tile.
+  __device__ __forceinline__ void fetch(int k0) {
+    vscale = scale;
+    vok = 0;
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
       const int k = k0 + kk[i];
-      dst[kk[i]][rr[i]] = (roff[i] >= 0 && k < K) ? scaled(base[roff[i] + (int64_t)k * ks], scale)
-                                                  : Num<T>::zero();
+      const bool ok = roff[i] >= 0 && k < K;
+      v[i] = base[ok ? roff[i] + (int64_t)k * ks : (int64_t)0];
+      vok |= (unsigned)ok << i;
     }
+  }
+  __device__ __forceinline__ void commit(T (*dst)[ROWS + 1]) const {
+#pragma unroll
+    for (int i = 0; i < NE; ++i)
+      dst[kk[i]][rr[i]] = ((vok >> i) & 1u) ? scaled(v[i], vscale) : Num<T>::zero();
   }
 };
 
@@ -520,15 +534,18 @@ __global__ __launch_bounds__(256) void shaped_gemm_kernel(
   int ks_cur = 0;  // k-steps held by the buffer being computed (0 = done)
   if (lp < un.p_end) {
     ZMPO_SETUP_PAIR(lp);
-    la.load(As[0], 0);
-    lb.load(Bs[0], 0);
+    la.fetch(0);
+    lb.fetch(0);
+    la.commit(As[0]);
+    lb.commit(Bs[0]);
     ks_cur = min(KT, lK);
     lkt = 1;
   }
   __syncthreads();
   int buf = 0;
   while (ks_cur > 0) {
-    // advance the load cursor and prefetch into the other buffer
+    // advance the load cursor and start the next tile's loads; they are
+    // committed to the other buffer after this tile's MFMAs
     int ks_nxt = 0;
     if (lkt * KT >= lK) {
       ++lp;
@@ -536,8 +553,8 @@ __global__ __launch_bounds__(256) void shaped_gemm_kernel(
       if (lp < un.p_end) ZMPO_SETUP_PAIR(lp);
     }
     if (lp < un.p_end) {
-      la.load(As[buf ^ 1], lkt * KT);
-      lb.load(Bs[buf ^ 1], lkt * KT);
+      la.fetch(lkt * KT);
+      lb.fetch(lkt * KT);
       ks_nxt = min(KT, lK - lkt * KT);
       ++lkt;
     }
@@ -561,6 +578,10 @@ __global__ __launch_bounds__(256) void shaped_gemm_kernel(
           }
         }
       }
+    }
+    if (ks_nxt > 0) {
+      la.commit(As[buf ^ 1]);
+      lb.commit(Bs[buf ^ 1]);
     }
     __syncthreads();
     buf ^= 1;
@@ -642,6 +663,326 @@ int launch_shaped(const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits
   return ZMPO_OK;
 }
 
+// -------------------------------------------------------------- panel tiles
+// Register-blocked grouped GEMM for large blocks (at D ~ 1000 the sigma
+// stage L x psi has M ~ 2500, N ~ K ~ 130 per pair; the stage x R has
+// M ~ N ~ 130 and hundreds of K ~ 2500 pairs per output block).  One
+// workgroup = one zmpo_unit of the shaped kernel's schedule (a TM x TN tile
+// over a range of pairs, split-K slots alike); its 4 waves form a 2 x 2 grid,
+// each owning (TM/2) x (TN/2) of the tile as BMW x BNW MFMA blocks, so every
+// operand fragment read from LDS feeds BNW (or BMW) MFMAs.  Operand tiles
+// are kept in LDS with k contiguous ([row][k], padded pitch) and a lane
+// fetches its whole fragment with one or two 16-byte reads: FP64 / FP32
+// lanes hold 4 consecutive k and feed four 16x16x4 MFMAs, BF16 lanes hold 8
+// and feed one 16x16x32 MFMA (the same relabelling of k on both operands
+// leaves every sum unchanged).  The next k-tile is loaded from global memory
+// into registers before the MFMAs of the current one and written to the
+// other LDS buffer after them: one barrier per k-tile.  k rows past a pair's
+// K and rows / columns past the block are zero in LDS, so the MFMAs need no
+// bounds tests.
+typedef __bf16 bfv8 __attribute__((ext_vector_type(8)));
+
+template <typename T>
+struct Panel;
+template <>
+struct Panel<double> {
+  static constexpr int KT = 16, KF = 4, PITCH = 18;
+  typedef dbl4 acc;
+  typedef double stg;  // register type of one staged element
+  struct frag {
+    double v[4];
+  };
+  __device__ static inline void read(frag& f, const double* p) {
+    const double2 x = *reinterpret_cast<const double2*>(p);
+    const double2 y = *reinterpret_cast<const double2*>(p + 2);
+    f.v[0] = x.x; f.v[1] = x.y; f.v[2] = y.x; f.v[3] = y.y;
+  }
+  __device__ static inline dbl4 mma(const frag& a, const frag& b, dbl4 c) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a.v[s], b.v[s], c, 0, 0, 0);
+    return c;
+  }
+  __device__ static inline int row(int g, int r) { return g + 4 * r; }
+};
+template <>
+struct Panel<float> {
+  static constexpr int KT = 16, KF = 4, PITCH = 20;
+  typedef flt4 acc;
+  typedef float stg;
+  struct frag {
+    float v[4];
+  };
+  __device__ static inline void read(frag& f, const float* p) {
+    const float4 x = *reinterpret_cast<const float4*>(p);
+    f.v[0] = x.x; f.v[1] = x.y; f.v[2] = x.z; f.v[3] = x.w;
+  }
+  __device__ static inline flt4 mma(const frag& a, const frag& b, flt4 c) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.v[s], b.v[s], c, 0, 0, 0);
+    return c;
+  }
+  __device__ static inline int row(int g, int r) { return 4 * g + r; }
+};
+template <>
+struct Panel<bf16s> {
+  static constexpr int KT = 32, KF = 8, PITCH = 40;
+  typedef flt4 acc;
+  typedef unsigned stg;  // one BF16 per 32-bit register (low half)
+  struct frag {
+    bfv8 v;
+  };
+  __device__ static inline void read(frag& f, const bf16s* p) {
+    f.v = __builtin_bit_cast(bfv8, *reinterpret_cast<const uint4*>(p));
+  }
+  __device__ static inline flt4 mma(const frag& a, const frag& b, flt4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int g, int r) { return 4 * g + r; }
+};
+
+// Staged element conversions (BF16 travels as the low half of a register).
+__device__ __forceinline__ double stg_get(double v, const double*) { return v; }
+__device__ __forceinline__ float stg_get(float v, const float*) { return v; }
+__device__ __forceinline__ unsigned stg_load(const bf16s* p) { return p->x; }
+__device__ __forceinline__ double stg_load(const double* p) { return *p; }
+__device__ __forceinline__ float stg_load(const float* p) { return *p; }
+__device__ __forceinline__ void stg_put(double* q, double v) { *q = v; }
+__device__ __forceinline__ void stg_put(float* q, float v) { *q = v; }
+__device__ __forceinline__ void stg_put(bf16s* q, unsigned v) { q->x = (unsigned short)v; }
+// v * scale; mode 0: scale 1 (unchanged), 1: scale -1 (exact sign flip),
+// 2: any other scale (BF16: product rounded to nearest even)
+__device__ __forceinline__ double stg_scale(double v, int mode, double sc) {
+  return mode == 0 ? v : mode == 1 ? -v : v * sc;
+}
+__device__ __forceinline__ float stg_scale(float v, int mode, float sc) {
+  return mode == 0 ? v : mode == 1 ? -v : v * sc;
+}
+__device__ __forceinline__ unsigned stg_scale(unsigned v, int mode, float sc) {
+  if (mode == 0) return v;
+  if (mode == 1) return v ^ 0x8000u;
+  return Num<bf16s>::put(__uint_as_float(v << 16) * sc).x;
+}
+
+// Per-thread load state of one operand tile (ROWS rows x KT k of one
+// pair).  Element i of thread t: for a k-contiguous operand (sk == 1) row
+// rf + i * rs at the fixed k = kb (16 / 32 consecutive threads cover a
+// row's k-tile), otherwise row rf at k = kb + i * ks (consecutive threads
+// cover consecutive rows).  off[i] is the element's offset at k-tile 0.
+template <typename T, int ROWS>
+struct PanelLoader {
+  static constexpr int KT = Panel<T>::KT;
+  static constexpr int NE = ROWS * KT / 256;
+  static_assert(NE >= 1 && NE <= 32, "panel tile too small or too large");
+  typedef typename Panel<T>::stg S;
+  typedef typename Num<T>::M M;
+  int64_t off[NE];
+  const T* base;
+  int64_t sk;
+  int K, kb, ks, rf, rs, mode;
+  unsigned okm;
+  unsigned vok;  // which fetched values are real elements
+  M scale;
+  __device__ __forceinline__ void setup(int tid, const T* b, int64_t boff, int K_, int64_t sk_,
+                                        int mlo, int64_t shi, int64_t slo, int r0, int rmax,
+                                        double sc) {
+    base = b + boff;
+    sk = sk_;
+    K = K_;
+    scale = (M)sc;
+    mode = sc == 1.0 ? 0 : sc == -1.0 ? 1 : 2;
+    if (sk_ == 1) {
+      kb = tid % KT; ks = 0; rf = tid / KT; rs = 256 / KT;
+    } else {
+      kb = tid / ROWS; ks = 256 / ROWS; rf = tid % ROWS; rs = 0;
+    }
+    const bool one = mlo >= rmax;  // single-level row map (uniform)
+    okm = 0;
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      const int r = r0 + rf + i * rs;
+      const bool ok = r < rmax;
+      int64_t o = 0;
+      if (ok) o = one ? (int64_t)r * slo : (int64_t)(r / mlo) * shi + (int64_t)(r % mlo) * slo;
+      off[i] = o + (int64_t)(kb + i * ks) * sk_;
+      okm |= (unsigned)ok << i;
+    }
+  }
+  // Loads are issued unconditionally (padding elements read the block's
+  // first element), so a tile's loads stay in flight through the MFMAs of
+  // the previous tile; store() zeroes padding and applies the pair's scale
+  // (the loader is not set up again between a fetch and its store).
+  __device__ __forceinline__ void fetch(S (&v)[NE], int k0) {
+    const int64_t ko = (int64_t)k0 * sk;
+    vok = 0;
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      const bool ok = ((okm >> i) & 1u) && (k0 + kb + i * ks < K);
+      v[i] = stg_load(base + (ok ? ko + off[i] : (int64_t)0));
+      vok |= (unsigned)ok << i;
+    }
+  }
+  __device__ __forceinline__ void store(T* lds, const S (&v)[NE]) const {
+    constexpr int P = Panel<T>::PITCH;
+#pragma unroll
+    for (int i = 0; i < NE; ++i)
+      stg_put(lds + (rf + i * rs) * P + kb + i * ks,
+              ((vok >> i) & 1u) ? stg_scale(v[i], mode, scale) : (S)0);
+  }
+};
+
+extern __shared__ __align__(16) unsigned char panel_lds[];
+
+template <typename T, typename TC, int TMv, int TNv>
+__global__ __launch_bounds__(256) void panel_gemm_kernel(
+    const zmpo_task* __restrict__ tasks, const zmpo_unit* __restrict__ units, int64_t nunits,
+    const zmpo_pair* __restrict__ pairs, const T* __restrict__ A, const T* __restrict__ B,
+    TC* __restrict__ C, TC* __restrict__ work, int beta) {
+  typedef Panel<T> PT;
+  typedef typename PT::acc accv;
+  constexpr int KT = PT::KT, P = PT::PITCH, KF = PT::KF;
+  constexpr int BMW = TMv / 32, BNW = TNv / 32;
+  static_assert(BMW >= 1 && BNW >= 1, "panel tile below 32 x 32");
+  T* const As0 = reinterpret_cast<T*>(panel_lds);
+  T* const Bs0 = As0 + 2 * TMv * P;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = wave_id();
+  const int lr = lane & 15, g = lane >> 4;
+  const int wm = wv & 1, wn = wv >> 1;
+  const int64_t u = xcd_logical_block(blockIdx.x, gridDim.x);
+  if (u >= nunits) return;  // workgroup-uniform
+  const zmpo_unit un = units[u];
+  const zmpo_task tk = tasks[un.task];
+  accv acc[BMW][BNW];
+#pragma unroll
+  for (int i = 0; i < BMW; ++i)
+#pragma unroll
+    for (int j = 0; j < BNW; ++j) acc[i][j] = accv{0, 0, 0, 0};
+  PanelLoader<T, TMv> la;
+  PanelLoader<T, TNv> lb;
+  typename PT::stg sa[PanelLoader<T, TMv>::NE], sb[PanelLoader<T, TNv>::NE];
+  int p = un.p_begin, k0 = 0;
+#define ZMPO_PANEL_SETUP(Q)                                                                      \
+  {                                                                                              \
+    const zmpo_pair pr = pairs[(Q)];                                                             \
+    la.setup(tid, A, pr.a_off, pr.K, pr.a_sk, pr.a_mlo, pr.a_smhi, pr.a_smlo, un.m0, tk.M,       \
+             pr.scale);                                                                          \
+    lb.setup(tid, B, pr.b_off, pr.K, pr.b_sk, pr.b_nlo, pr.b_snhi, pr.b_snlo, un.n0, tk.N, 1.0); \
+  }
+  bool have = p < un.p_end;
+  if (have) {
+    ZMPO_PANEL_SETUP(p);
+    la.fetch(sa, 0);
+    lb.fetch(sb, 0);
+    la.store(As0, sa);
+    lb.store(Bs0, sb);
+  }
+  __syncthreads();
+  int cur = 0;
+  while (have) {
+    // advance to the next (pair, k-tile) and start its global loads
+    k0 += KT;
+    if (k0 >= la.K) {
+      ++p;
+      k0 = 0;
+      if (p < un.p_end) ZMPO_PANEL_SETUP(p);
+    }
+    const bool nxt = p < un.p_end;
+    if (nxt) {
+      la.fetch(sa, k0);
+      lb.fetch(sb, k0);
+    }
+    const T* __restrict__ As = As0 + cur * TMv * P;
+    const T* __restrict__ Bs = Bs0 + cur * TNv * P;
+    typename PT::frag fa[BMW], fb[BNW];
+#pragma unroll
+    for (int i = 0; i < BMW; ++i) PT::read(fa[i], As + (wm * (TMv / 2) + i * 16 + lr) * P + KF * g);
+#pragma unroll
+    for (int j = 0; j < BNW; ++j) PT::read(fb[j], Bs + (wn * (TNv / 2) + j * 16 + lr) * P + KF * g);
+#pragma unroll
+    for (int i = 0; i < BMW; ++i)
+#pragma unroll
+      for (int j = 0; j < BNW; ++j) acc[i][j] = PT::mma(fa[i], fb[j], acc[i][j]);
+    if (nxt) {
+      la.store(As0 + (cur ^ 1) * TMv * P, sa);
+      lb.store(Bs0 + (cur ^ 1) * TNv * P, sb);
+    }
+    __syncthreads();
+    cur ^= 1;
+    have = nxt;
+  }
+#undef ZMPO_PANEL_SETUP
+  typedef typename Num<TC>::M MC;
+  if (un.slot >= 0) {
+    TC* __restrict__ w = work + (int64_t)un.slot * (TMv * TNv);
+#pragma unroll
+    for (int i = 0; i < BMW; ++i)
+#pragma unroll
+      for (int j = 0; j < BNW; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = wm * (TMv / 2) + i * 16 + PT::row(g, r);
+          const int col = wn * (TNv / 2) + j * 16 + lr;
+          w[row * TNv + col] = Num<TC>::put((MC)acc[i][j][r]);
+        }
+    return;
+  }
+  const int mv = min(TMv, tk.M - un.m0);
+  const int nv = min(TNv, tk.N - un.n0);
+  const bool one = tk.c_mlo >= tk.M;  // single-level row map: no division
+  TC* __restrict__ cb = C + tk.c_off;
+#pragma unroll
+  for (int i = 0; i < BMW; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = wm * (TMv / 2) + i * 16 + PT::row(g, r);
+      if (row >= mv) continue;
+      const int mi = un.m0 + row;
+      const int64_t ro = one ? (int64_t)mi * tk.c_smlo
+                             : (int64_t)(mi / tk.c_mlo) * tk.c_smhi + (int64_t)(mi % tk.c_mlo) * tk.c_smlo;
+#pragma unroll
+      for (int j = 0; j < BNW; ++j) {
+        const int col = wn * (TNv / 2) + j * 16 + lr;
+        if (col < nv) store_acc(cb + ro + (int64_t)(un.n0 + col) * tk.c_sn, acc[i][j][r], beta);
+      }
+    }
+}
+
+template <typename T, typename TC, int TMv, int TNv>
+int launch_panel(const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
+                 const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
+                 const void* A, const void* B, void* C, void* work, int beta, hipStream_t s) {
+  const size_t lds = (size_t)2 * (TMv + TNv) * Panel<T>::PITCH * sizeof(T);
+  if (nunits > 0) {
+    if (lds > 64 * 1024) {
+      // raised once per device, before the first launch that needs it
+      constexpr int MAXDEV = 64;
+      static int allowed[MAXDEV] = {0};
+      int dev = 0;
+      int rc = check_hip(hipGetDevice(&dev), "panelgemm: device");
+      if (rc) return rc;
+      if (dev < 0 || dev >= MAXDEV) return set_error(ZMPO_ERR_ARG, "panelgemm: device index beyond 63");
+      if (!allowed[dev]) {
+        rc = check_hip(hipFuncSetAttribute((const void*)panel_gemm_kernel<T, TC, TMv, TNv>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                       "panelgemm: LDS attribute");
+        if (rc) return rc;
+        allowed[dev] = 1;
+      }
+    }
+    hipLaunchKernelGGL((panel_gemm_kernel<T, TC, TMv, TNv>), dim3((unsigned)nunits), dim3(256), lds,
+                       s, tasks, units, nunits, pairs, (const T*)A, (const T*)B, (TC*)C, (TC*)work,
+                       beta);
+    int rc = check_hip(hipGetLastError(), "panel_gemm launch");
+    if (rc) return rc;
+  }
+  if (ngroups > 0) {
+    hipLaunchKernelGGL((shaped_reduce_kernel<TC, TMv, TNv>), dim3((unsigned)ngroups), dim3(256), 0,
+                       s, tasks, groups, ngroups, (const TC*)work, (TC*)C, beta);
+    return check_hip(hipGetLastError(), "panel_reduce launch");
+  }
+  return ZMPO_OK;
+}
+
 template <typename T, typename TC = T>
 int dispatch_shaped(int shape, const zmpo_task* tasks, const zmpo_unit* units, int64_t nunits,
                     const zmpo_rgroup* groups, int64_t ngroups, const zmpo_pair* pairs,
@@ -653,6 +994,12 @@ int dispatch_shaped(int shape, const zmpo_task* tasks, const zmpo_unit* units, i
     case ZMPO_TILE_64x32: return launch_shaped<T, TC, 64, 32>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
     case ZMPO_TILE_16x16: return launch_shaped<T, TC, 16, 16>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
     case ZMPO_TILE_64x64: return launch_shaped<T, TC, 64, 64>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_PANEL_64x64: return launch_panel<T, TC, 64, 64>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_PANEL_128x64: return launch_panel<T, TC, 128, 64>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_PANEL_64x128: return launch_panel<T, TC, 64, 128>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_PANEL_128x128: return launch_panel<T, TC, 128, 128>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_PANEL_32x64: return launch_panel<T, TC, 32, 64>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_PANEL_64x32: return launch_panel<T, TC, 64, 32>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
     default: return set_error(ZMPO_ERR_ARG, "shapedgemm: unknown tile shape");
   }
 }
@@ -682,6 +1029,8 @@ struct StreamLoader {
   bool kmaj;   // element layout of the pair set up last
   bool vkmaj;  // element layout of the values held in v (set by fetch)
   M scale;
+  M vscale;      // scale of the pair whose values are held in v
+  unsigned vok;  // which of v are real elements (the rest read the block's first)
   __device__ __forceinline__ static int kidx(bool km, int e) { return km ? e % KT : e / ROWS; }
   __device__ __forceinline__ static int ridx(bool km, int e) { return km ? e / KT : e % ROWS; }
   __device__ __forceinline__ void setup(int tid, const T* b, int64_t off, int K_, int64_t sk,
@@ -700,22 +1049,30 @@ struct StreamLoader {
     }
   }
   // The cursor may set up the next pair before the fetched values are
-  // stored, so store() uses the layout recorded here.
+  // stored, so store() uses the layout, scale and validity recorded here.
+  // Every lane issues its loads unconditionally (padding elements read the
+  // block's first element), so the loads of a tile are all in flight while
+  // the MFMAs of the previous one run; zeroing and scaling wait for store().
   __device__ __forceinline__ void fetch(int tid, int k0) {
     vkmaj = kmaj;
+    vscale = scale;
+    vok = 0;
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
       const int e = tid + 256 * i;
       const int k = k0 + kidx(kmaj, e);
-      v[i] = (roff[i] >= 0 && k < K) ? scaled(base[roff[i] + (int64_t)k * ks], scale)
-                                     : Num<T>::zero();
+      const bool ok = roff[i] >= 0 && k < K;
+      v[i] = base[ok ? roff[i] + (int64_t)k * ks : (int64_t)0];
+      vok |= (unsigned)ok << i;
     }
   }
   __device__ __forceinline__ void store(int tid, T (*dst)[ROWS + 1]) const {
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
       const int e = tid + 256 * i;
-      if (e < ROWS * KT) dst[kidx(vkmaj, e)][ridx(vkmaj, e)] = v[i];
+      if (e < ROWS * KT)
+        dst[kidx(vkmaj, e)][ridx(vkmaj, e)] = ((vok >> i) & 1u) ? scaled(v[i], vscale)
+                                                                : Num<T>::zero();
     }
   }
 };
@@ -761,6 +1118,8 @@ __global__ __launch_bounds__(256) void stream_gemm_kernel(
 #pragma unroll
   for (int i = 0; i < lb.NE; ++i) lb.v[i] = Num<T>::zero();
   la.vkmaj = lb.vkmaj = true;
+  la.vok = lb.vok = 0;
+  la.vscale = lb.vscale = (M)1;
   accv acc[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) acc[i] = accv{0, 0, 0, 0};
@@ -1270,9 +1629,13 @@ struct ChainRows {
 };
 
 // Operands of the k-steps [k0, k0 + 4*KS) of one pair, 32-column tile
-// n0.  Only k-steps that start below K are loaded (uniform test); lanes past
-// K inside the last k-step read the zero padding of both operands.  Bases
-// are wave-uniform (scalar registers); each lane adds a fixed 32-bit offset.
+// n0.  Every slice issues the same loads: k-steps at or past K re-read the
+// last k-step that starts below K (their MFMAs are skipped, or their
+// operands zeroed at use for BF16), so the number of loads in flight is
+// fixed and the MFMAs of the previous slice wait only for its own operands.
+// Lanes past K inside the last k-step read the zero padding of both
+// operands.  Bases are wave-uniform (scalar registers); each lane adds a
+// fixed 32-bit offset.
 template <typename T, int RB, bool GA, bool N16>
 __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPair& pr, int n0,
                                            int k0, const T* __restrict__ src,
@@ -1283,6 +1646,7 @@ __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPai
   constexpr int LD = 16 * RB + 1;
   const int lane_b = lk * pr.ldb + lr;
   const T* __restrict__ b0 = Bb + pr.b_off + (int64_t)k0 * pr.ldb + n0;
+  const int smax = (pr.K - 1 - k0) >> 2;  // last k-step of the slice below K (k0 < K)
   if (GA) {
     // A block in global memory: lane rows fixed, k clamped into the block
     // (columns past K meet zero B rows)
@@ -1293,37 +1657,26 @@ __device__ __forceinline__ void chain_load(ChainSlice<T, RB>& sl, const ChainPai
     for (int i = 0; i < RB; ++i) ro[i] = (int64_t)rows.mb[i] * bk.a_smhi + (int64_t)rows.mr[i] * bk.a_smlo;
 #pragma unroll
     for (int s = 0; s < ChainSlice<T, RB>::KS; ++s) {
-      if (k0 + 4 * s < pr.K) {
-        const T* __restrict__ bs = b0 + 4 * s * pr.ldb;
-        sl.b[s][0] = bs[lane_b];
-        if (!N16) sl.b[s][1] = bs[lane_b + 16];
-        const int64_t ko = (int64_t)min(k0 + 4 * s + lk, pr.K - 1) * bk.a_sk;
+      const int sc = min(s, smax);
+      const T* __restrict__ bs = b0 + 4 * sc * pr.ldb;
+      sl.b[s][0] = bs[lane_b];
+      if (!N16) sl.b[s][1] = bs[lane_b + 16];
+      const int64_t ko = (int64_t)min(k0 + 4 * sc + lk, pr.K - 1) * bk.a_sk;
 #pragma unroll
-        for (int i = 0; i < RB; ++i) sl.a[s][i] = ab[ro[i] + ko];
-      } else if (Acc<T>::KG > 1) {
-        // k-steps past K inside an MFMA k block contribute zero
-        sl.b[s][0] = sl.b[s][1] = Num<T>::zero();
-#pragma unroll
-        for (int i = 0; i < RB; ++i) sl.a[s][i] = Num<T>::zero();
-      }
+      for (int i = 0; i < RB; ++i) sl.a[s][i] = ab[ro[i] + ko];
     }
     return;
   }
   const T* __restrict__ a0 = src + (pr.c + k0) * LD;
 #pragma unroll
   for (int s = 0; s < ChainSlice<T, RB>::KS; ++s) {
-    if (k0 + 4 * s < pr.K) {
-      const T* __restrict__ bs = b0 + 4 * s * pr.ldb;
-      sl.b[s][0] = bs[lane_b];
-      if (!N16) sl.b[s][1] = bs[lane_b + 16];
-      const T* __restrict__ as = a0 + 4 * s * LD;
+    const int sc = min(s, smax);
+    const T* __restrict__ bs = b0 + 4 * sc * pr.ldb;
+    sl.b[s][0] = bs[lane_b];
+    if (!N16) sl.b[s][1] = bs[lane_b + 16];
+    const T* __restrict__ as = a0 + 4 * sc * LD;
 #pragma unroll
-      for (int i = 0; i < RB; ++i) sl.a[s][i] = as[lane_a + i * 16];
-    } else if (Acc<T>::KG > 1) {
-      sl.b[s][0] = sl.b[s][1] = Num<T>::zero();
-#pragma unroll
-      for (int i = 0; i < RB; ++i) sl.a[s][i] = Num<T>::zero();
-    }
+    for (int i = 0; i < RB; ++i) sl.a[s][i] = as[lane_a + i * 16];
   }
 }
 
@@ -1408,12 +1761,16 @@ struct ChainWalk {
       for (int i = 0; i < RB; ++i)
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
-          if constexpr (KG == 1)
+          if constexpr (KG == 1) {
             acc[i][j] = Acc<T>::mfma(sl.a[s][i], sl.b[s][j], acc[i][j]);
-          else
-            acc[i][j] = Acc<T>::mfma4(sl.a[s][i], sl.a[s + 1][i], sl.a[s + 2][i], sl.a[s + 3][i],
-                                      sl.b[s][j], sl.b[s + 1][j], sl.b[s + 2][j], sl.b[s + 3][j],
-                                      acc[i][j]);
+          } else {
+            // k-steps at or past ks (re-read operands) contribute zero
+            const T z = Num<T>::zero();
+            acc[i][j] = Acc<T>::mfma4(
+                sl.a[s][i], s + 1 < ks ? (T)sl.a[s + 1][i] : z, s + 2 < ks ? (T)sl.a[s + 2][i] : z,
+                s + 3 < ks ? (T)sl.a[s + 3][i] : z, sl.b[s][j], s + 1 < ks ? (T)sl.b[s + 1][j] : z,
+                s + 2 < ks ? (T)sl.b[s + 2][j] : z, s + 3 < ks ? (T)sl.b[s + 3][j] : z, acc[i][j]);
+          }
         }
     }
   }

@@ -27,6 +27,7 @@ import os
 import numpy as np
 
 from . import _lib
+from . import autotune
 from . import device as dv
 from .qtensor import (ChainProgram, Program, QTensor, _workspace, chain_structure, gather,
                       run_chain, run_program, seq_add_chain, seq_add_program)
@@ -389,6 +390,10 @@ class SigmaProgram:
         if not self.stages or all(st[3].ntasks == 0 for st in self.stages):
             sigma[:] = 0
             return sigma
+        tuned = self.__dict__.setdefault('_tuned', set())
+        if dtype not in tuned and getattr(psi, 'is_cuda', False):
+            self._choose_tiles(L, R, psi, sigma, dtype, t1, t3)
+            tuned.add(dtype)
         if fused and not self.fused:
             raise ValueError('SigmaProgram built without the chained stages (fused=False)')
         if fused and overlap and len(self.stages) > 1 and not isinstance(psi, np.ndarray) \
@@ -419,6 +424,18 @@ class SigmaProgram:
                 run_program(st[3], t3, R, sigma, out_dtype(dtype), beta=0 if first else 1)
                 first = False
         return sigma
+
+    def _choose_tiles(self, L, R, psi, sigma, dtype, t1, t3):
+        """Tiles of the L x psi and x R launches in this dtype (autotune.py:
+        the measured table, or tuning on the device in 'online' mode, which
+        writes t1 and sigma before the sigma proper overwrites them)."""
+        odt = out_dtype(dtype)
+        for st in self.stages:
+            autotune.choose(st[0], 'sigma_l', dtype,
+                            lambda pg=st[0]: run_program(pg, L, psi, t1, dtype))
+            if st[3].ntasks:
+                autotune.choose(st[3], 'sigma_r', odt,
+                                lambda pg=st[3]: run_program(pg, t3, R, sigma, odt, beta=0))
 
     def _apply_overlapped(self, mpo, L, R, psi, sigma, dtype, t1, t3):
         """Chunked sigma with L x psi of chunk c+1 on a side stream while the

@@ -555,7 +555,18 @@ def make_schedule(tasks, pairs, rb, kchunk=2048, reuse_key=None, tile=None):
     return units, groups, int(split.sum())
 
 
-SHAPES = {0: (64, 16), 1: (128, 16), 2: (32, 32), 3: (64, 32), 4: (16, 16), 5: (64, 64)}
+SHAPES = {0: (64, 16), 1: (128, 16), 2: (32, 32), 3: (64, 32), 4: (16, 16), 5: (64, 64),
+          # register-blocked panel tiles (zmpo_shapedgemm ZMPO_PANEL_*): same
+          # schedule, 2x2 waves, k-contiguous LDS, BF16 on the 16x16x32 MFMA
+          6: (64, 64), 7: (128, 64), 8: (64, 128), 9: (128, 128), 10: (32, 64), 11: (64, 32)}
+PANEL_SHAPES = (6, 7, 8, 9, 10, 11)
+# summed K per unit of the panel tiles: each unit of a long reduction (sigma
+# x R: hundreds of K ~ 2500 pairs per output block) runs ~10 pairs, which
+# keeps the split-K workspace to a few GB at K = 36, D = 1000
+PANEL_KCHUNK = int(os.environ.get('ZMPO_PANEL_KCHUNK', '32768'))
+# shapes the static cost model chooses among (the panel tiles are chosen by
+# measurement, autotune.py)
+MODEL_SHAPES = (0, 1, 2, 3, 4, 5)
 
 
 def choose_shape(tasks, pairs):
@@ -568,7 +579,8 @@ def choose_shape(tasks, pairs):
     kpad = np.bincount(rows, weights=((pairs['K'].astype(np.int64) + 15) // 16) * 16,
                        minlength=len(tasks)) if len(pairs) else np.zeros(len(tasks))
     best, cost_best = 0, None
-    for sid, (tm_, tn_) in SHAPES.items():
+    for sid in MODEL_SHAPES:
+        tm_, tn_ = SHAPES[sid]
         ntile = np.maximum((M + tm_ - 1) // tm_, 1) * np.maximum((N + tn_ - 1) // tn_, 1)
         cost = float(np.sum(ntile * (tm_ * tn_ * np.maximum(kpad, 16) + 4096)))
         if cost_best is None or cost < cost_best:
@@ -669,6 +681,10 @@ class Program:
         self.kt = None    # stream kernel k-tile (None: library default)
         self.kt_dtype = None   # stream kernel k-tile per dtype code (overrides kt)
         self.upw = None   # stream kernel units per workgroup (None: by launch size)
+        self.kchunk = None   # summed K per split-K unit (None: by kernel)
+        # per kernel dtype code: (kernel, tile shape) replacing (prefer, shape)
+        # for launches in that type (set by the autotuner)
+        self.tiles = {}
 
     # The device-ready arrays and the schedule choices are derived on first
     # use (a sigma plan builds programs that only the unfused path launches).
@@ -773,17 +789,37 @@ class Program:
             sc.pop('units', None)
             sc.pop('groups', None)
 
-    def schedule(self, kernel='wave'):
-        """Device units / split-K groups for the wave or shaped kernel (built once each)."""
+    def launch_choice(self, dtype, kernel=None):
+        """(kernel, tile shape) of a launch in kernel dtype code ``dtype``: an
+        explicit kernel keeps the program's shape; otherwise the per-dtype
+        choice (autotuner) or the program's preference.  Kernels that cannot
+        run the program fall back to the shaped one."""
+        shape = self.shape
+        if kernel is None:
+            kernel, shape = self.tiles.get(dtype, (self.prefer or KERNEL, self.shape))
+        if kernel == 'stream' and (not self.small_blocks or shape in PANEL_SHAPES):
+            kernel = 'shaped'
+        if kernel == 'skinny' and not self.skinny_ok():
+            kernel = 'shaped'
+        return kernel, shape
+
+    def schedule(self, kernel='wave', shape=None):
+        """Device units / split-K groups for one kernel and tile shape (built
+        once each; shape None: the program's shape)."""
+        shape = self.shape if shape is None else shape
         if self._sched is None:
             self._sched = {}
-        if kernel == 'skinny' and kernel not in self._sched:
+        key = (kernel, shape)
+        if kernel == 'skinny' and key not in self._sched:
             u = self.skinny_units(SKINNY_RB)
-            self._sched[kernel] = {'units_dev_host': u, 'nunits': len(u), 'rb': SKINNY_RB}
-        if kernel not in self._sched:
-            tile = None if kernel == 'wave' else (32, 32) if kernel == 'dot' else SHAPES[self.shape]
+            self._sched[key] = {'units_dev_host': u, 'nunits': len(u), 'rb': SKINNY_RB}
+        if key not in self._sched:
+            tile = None if kernel == 'wave' else (32, 32) if kernel == 'dot' else SHAPES[shape]
+            kchunk = self.kchunk or (PANEL_KCHUNK if (kernel == 'shaped' and
+                                                      shape in PANEL_SHAPES) else 2048)
             units, groups, nslots = make_schedule(self.tasks_host, self.pairs_host, self.rb,
-                                                  reuse_key=self.reuse_key, tile=tile)
+                                                  kchunk=kchunk, reuse_key=self.reuse_key,
+                                                  tile=tile)
             size = self.rb * 256 if kernel == 'wave' else tile[0] * tile[1]
             sc = {'nunits': len(units), 'ngroups': len(groups), 'nslots': nslots,
                   'slot_size': size, 'units_host': units, 'groups_host': groups}
@@ -794,8 +830,8 @@ class Program:
                 sc['upw'] = self.upw or STREAM_UPW or stream_upw(len(su))
             else:
                 sc['units_dev_host'] = units
-            self._sched[kernel] = sc
-        sc = self._sched[kernel]
+            self._sched[key] = sc
+        sc = self._sched[key]
         if 'units' not in sc:
             sc['units'] = dv.upload_bytes(sc['units_dev_host'])
         if 'groups' not in sc and 'groups_host' in sc:
@@ -827,12 +863,7 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None)
     'tile' runs the 64x64 LDS-tiled workgroup kernel."""
     if isinstance(prog, dict):
         prog = Program(prog['tasks'], prog['pairs'])
-    if kernel is None:
-        kernel = prog.prefer if prog.prefer else KERNEL
-    if kernel == 'stream' and not prog.small_blocks:
-        kernel = 'shaped'
-    if kernel == 'skinny' and not prog.skinny_ok():
-        kernel = 'shaped'
+    kernel, shape = prog.launch_choice(dtype, kernel)
     if prog.ntasks == 0:
         return
     tasks, pairs = prog.device()
@@ -842,7 +873,7 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None)
         _lib.check(L.zmpo_blockgemm(dtype, dv.ptr(tasks), prog.ntasks, dv.ptr(pairs),
                                     prog.ntiles, dv.ptr(A), dv.ptr(B), dv.ptr(C), beta, st))
         return
-    sc = prog.schedule(kernel)
+    sc = prog.schedule(kernel, shape)
     work = _workspace(sc['nslots'] * sc['slot_size'], dtype) if sc.get('nslots') else None
     if kernel == 'dot':
         _lib.check(L.zmpo_dotgemm(dtype, dv.ptr(tasks), dv.ptr(sc['units']), sc['nunits'],
@@ -856,7 +887,7 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None)
         return
     if kernel == 'stream':
         kt = prog.kt_dtype.get(dtype, sc['kt']) if prog.kt_dtype else sc['kt']
-        _lib.check(L.zmpo_streamgemm(dtype, prog.shape, kt, dv.ptr(sc['units']),
+        _lib.check(L.zmpo_streamgemm(dtype, shape, kt, dv.ptr(sc['units']),
                                      sc['nunits'], sc['upw'], dv.ptr(tasks), dv.ptr(sc['groups']),
                                      sc['ngroups'], dv.ptr(pairs), dv.ptr(A), dv.ptr(B),
                                      dv.ptr(C), dv.ptr(work), beta, st))
@@ -866,21 +897,16 @@ def run_program(prog, A, B, C, dtype=_lib.F64, beta=0, stream=None, kernel=None)
                                    dv.ptr(pairs), dv.ptr(A), dv.ptr(B), dv.ptr(C), dv.ptr(work),
                                    beta, st))
     else:
-        _lib.check(L.zmpo_shapedgemm(dtype, prog.shape, dv.ptr(tasks), dv.ptr(sc['units']),
+        _lib.check(L.zmpo_shapedgemm(dtype, shape, dv.ptr(tasks), dv.ptr(sc['units']),
                                      sc['nunits'], dv.ptr(sc['groups']), sc['ngroups'],
                                      dv.ptr(pairs), dv.ptr(A), dv.ptr(B), dv.ptr(C),
                                      dv.ptr(work), beta, st))
 
 
-def program_kernel(prog, kernel=None):
-    """The grouped-GEMM kernel run_program would use for prog."""
-    if kernel is None:
-        kernel = prog.prefer if prog.prefer else KERNEL
-    if kernel == 'stream' and not prog.small_blocks:
-        kernel = 'shaped'
-    if kernel == 'skinny' and not prog.skinny_ok():
-        kernel = 'shaped'
-    return kernel
+def program_kernel(prog, kernel=None, dtype=None):
+    """The grouped-GEMM kernel run_program would use for prog (in kernel
+    dtype code ``dtype``)."""
+    return prog.launch_choice(dtype, kernel)[0]
 
 
 def seq_add_program(seq, prog, dtype, a, b, c, work, beta=0, out_f32=False):
@@ -890,14 +916,14 @@ def seq_add_program(seq, prog, dtype, a, b, c, work, beta=0, out_f32=False):
     (only the shaped and stream kernels can)."""
     if prog.ntasks == 0:
         return 0
-    kernel = program_kernel(prog)
+    kernel, shape = prog.launch_choice(dtype)
     if kernel not in ('shaped', 'stream'):
         return None
     tasks, pairs = prog.device()
-    sc = prog.schedule(kernel)
+    sc = prog.schedule(kernel, shape)
     rec = np.zeros(1, _lib.SEQ_OP_DTYPE)
     rec['kind'] = _lib.SEQ_STREAM if kernel == 'stream' else _lib.SEQ_SHAPED
-    rec['shape'] = prog.shape
+    rec['shape'] = shape
     if kernel == 'stream':
         kt = prog.kt_dtype.get(dtype, sc['kt']) if prog.kt_dtype else sc['kt']
         rec['kt'], rec['upw'] = kt, sc['upw']

@@ -123,3 +123,70 @@ def synthetic_mps(K, nelec, D, upto_left, from_right, seed=0, sN=None, sS=None):
         rsites[i] = _orth_site(qt, rng, False)
         dims = nxt
     return lsites, rsites, qnuml, qnumr
+
+
+# (N, Sz) sector widths of the synthetic MPS.  'dmrg' matches the middle
+# bond of a real K = 36, D = 1000 DMRG state (the first D = 1000 sweep of a
+# Normal2 ramp from the product state: 24 / 21 sectors, ndim0 1.20e6, 5.7
+# TFLOP per sigma; synthetic: 28 / 28 sectors, ndim0 1.18e6); 'wide' is the
+# generator's default, with many small sectors (116 at K = 36, ndim0 2.8e5).
+STRUCTURES = {'dmrg': lambda K: (0.5 + 0.02 * K, 0.35 + 0.01 * K), 'wide': lambda K: (None, None)}
+
+
+def middle_bond(K, D, seed, budget, fused=True, extra=None, structure='dmrg', log=None):
+    """Middle-bond two-site sigma workload of a K-orbital, N = K, Sz = 0
+    problem at bond dimension D: seeded integrals, the device MPO, a
+    synthetic MPS (sector widths by ``structure``) and its left / right
+    environments built on the device by the renormalisation launches.
+    Returns (mpo, isite, st, L, R, prog) with st.qtmp / st.qkey the psi
+    template and prog its SigmaProgram.  ``extra`` (a dict) also receives
+    the site tensors, bond quantum numbers and the environments one site
+    before L and R with the site tensors that renormalise them."""
+    import time
+    from . import device as dv
+    from . import kernelQt as kq
+    from . import mpo as pmpo
+    from .dot import setupQtmpCI
+    from .integrals import gen_integrals
+    log = log or (lambda msg: None)
+    t0 = time.time()
+    i1, i2 = gen_integrals(K, seed=seed)
+    mpo = pmpo.MPOStore(i1, i2)
+    log('MPO K=%d built in %.1fs' % (K, time.time() - t0))
+    isite = K // 2 - 1
+    sN, sS = STRUCTURES[structure](K)
+    ls, rs, ql, qr = synthetic_mps(K, K, D, isite - 1, isite + 2, seed=seed, sN=sN, sS=sS)
+    if extra is not None:
+        extra.update(ls=ls, rs=rs, ql=ql, qr=qr)
+    log('synthetic MPS: bond dims L %s R %s' % (len(ql[isite]), len(qr[isite + 2])))
+    ren_flops = 0
+    t = time.time()
+    env = kq.boundary_store(mpo.nops)
+    for i in range(isite):
+        if extra is not None and i == isite - 1:
+            extra.update(Lprev=env, lsite=ls[i], lidx=i)
+        env, prog = kq.renorm(mpo, i, ls[i], env, 'L')
+        ren_flops += prog.flops
+    L = env
+    env = kq.boundary_store(mpo.nops)
+    for i in range(K - 1, isite + 1, -1):
+        if extra is not None and i == isite + 2:
+            extra.update(Rprev=env, rsite=rs[i], ridx=i)
+        env, prog = kq.renorm(mpo, i, rs[i], env, 'R')
+        ren_flops += prog.flops
+    R = env
+    dv.sync()
+    log('environments built (%d renormalisations, %.1f GFLOP, %.1fs incl. plans)' % (
+        K - 2, ren_flops / 1e9, time.time() - t))
+
+    class St:
+        pass
+    st = St()
+    st.qlst = [ql[isite], pmpo.qphys(), pmpo.qphys(), qr[isite + 2]]
+    setupQtmpCI(st, (float(K), 0.0))
+    t = time.time()
+    prog = kq.SigmaProgram(mpo, isite, st.qtmp, st.qkey, L, R, mpo.hpwts, budget=budget,
+                           fused=fused)
+    log('sigma plan: ndim0=%d flops=%.3e chunks=%d (%.1fs)' % (
+        st.qtmp.size_allowed, prog.flops, len(prog.chunks), time.time() - t))
+    return mpo, isite, st, L, R, prog
