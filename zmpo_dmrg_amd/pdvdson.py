@@ -112,6 +112,9 @@ class EigenSolver:
         # symmetry check of the projected H: sigma applications in FP32
         # leave an asymmetry at the FP32 rounding level
         self.herm_tol = 1e-10
+        # residual criterion max(crit_vec, tol_scale * max(1, |e|)): the
+        # level a low-precision sigma still resolves scales with |H psi|
+        self.tol_scale = 0.0
 
     def _ortho(self, ops, V, nb, R, rows, out_start):
         """Orthonormalise R[rows] against V[:nb] (5 passes) into V[out_start:]."""
@@ -195,7 +198,7 @@ class EigenSolver:
                 ops.lincomb(R[i], W, vr[i], 1.0, 0.0)
                 ops.lincomb(R[i], V, vr[i], -eigs[i], 1.0)
                 rn = ops.norm(R[i])
-                rconv.append(rn <= self.crit_vec)
+                rconv.append(rn <= max(self.crit_vec, self.tol_scale * max(1.0, abs(eigs[i]))))
                 if not rconv[i] and not econv[i]:
                     rindx.append(i)
             if not rindx or niter == self.maxcycle - 1:
