@@ -107,6 +107,22 @@ typedef struct zmpo_rgroup {
   int32_t task, m0, n0, slot_begin, nslots, pad;
 } zmpo_rgroup;
 
+/* Host-side schedule builder for the grouped-GEMM launches (host arrays
+ * only, no device work): the units of a TM x TN tiling of every task, one
+ * per (tile, chunk of the task's pairs), chunks cut where the summed K of
+ * the task's preceding pairs crosses a multiple of kchunk; order (task,
+ * tile, chunk), or (key, chunk, task, tile) with a per-task reuse_key
+ * (NULL: none); split-K slots numbered in (task, tile, chunk) order over
+ * the split tasks; one zmpo_rgroup per split tile, in unit order.  With
+ * units == NULL only counts[0..2] = (units, groups, slots) are set.  Replaces
+ * the per-pair Python loop that drives the reference's block products
+ * (qtensor.tensordotCAL, source/qtensor/qtensor.py:642-659) with work lists
+ * built once per plan. */
+int zmpo_schedule_build(int64_t nt, const zmpo_task* tasks, const zmpo_pair* pairs, int tm,
+                        int tn, int64_t kchunk, const int64_t* reuse_key, zmpo_unit* units,
+                        int64_t units_cap, zmpo_rgroup* groups, int64_t groups_cap,
+                        int64_t* counts);
+
 const char* zmpo_last_error(void);
 int zmpo_abi_version(void);
 int zmpo_struct_sizes(int* task_size, int* pair_size);
@@ -311,6 +327,17 @@ int zmpo_scatter(int dtype_dst, void* dst, int dtype_src, const void* src, const
                  int64_t n, void* stream);
 int zmpo_convert(int dtype_dst, void* dst, int dtype_src, const void* src, int64_t n,
                  void* stream);
+
+/* Truncation eigensolver: eigenvalues (ascending) and eigenvectors of the
+ * symmetric density-matrix sectors of one decimation (the per-sector eigh of
+ * mpo_dmrg_qparser.py:216-293 / rdm_blkdiag, scipy/numpy eigh in the
+ * reference), FP64 on the device by rocSOLVER dsyevd on `stream`.  Sector s
+ * is n[s] x n[s] at A + off[s]; eigenvector j overwrites A[off[s] + j*n[s]
+ * ...]; eigenvalues go to W + woff[s].  Host arrays n, off, woff; device A,
+ * W, E (>= max n[s] doubles), info (>= nsec ints; nonzero: no convergence,
+ * read after the stream completes). */
+int zmpo_syevd_sectors(int nsec, const int64_t* n, const int64_t* off, const int64_t* woff,
+                       double* A, double* W, double* E, int* info, void* stream);
 
 /* Device store: allocation and copies of the buffers the launches read and
  * write (MPO sites, environments, psi / sigma, intermediates).  No

@@ -172,7 +172,8 @@ def test_device_dmrg_matches_oracle_fixture(gpu, name):
         assert np.array_equal(np.asarray(d.qnumr[i]), z['qnumr%d' % i]), i
 
 
-@pytest.mark.parametrize('mode,lo', [('correction', 1), ('correction', 2), ('restart', 1)])
+@pytest.mark.parametrize('mode,lo', [('rq', 1), ('rq', 2), ('correction', 1), ('correction', 2),
+                                     ('restart', 1)])
 def test_fp32_sigma_with_fp64_correction_matches_fp64(gpu, mode, lo):
     """Precision form: low-precision sigma (FP32 or BF16 operands) with FP64
     correction -- 'correction': outer FP64 sigmas around anchored inner

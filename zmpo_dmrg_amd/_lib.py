@@ -58,7 +58,7 @@ EXPORTS = ['zmpo_last_error', 'zmpo_abi_version', 'zmpo_struct_sizes', 'zmpo_sym
            'zmpo_symAllowedInt', 'zmpo_blockgemm', 'zmpo_wavegemm', 'zmpo_shapedgemm', 'zmpo_streamgemm', 'zmpo_chaingemm', 'zmpo_chaingemm16', 'zmpo_skinnygemm', 'zmpo_dotgemm', 'zmpo_dots', 'zmpo_dots_workspace',
            'zmpo_lincomb', 'zmpo_precond', 'zmpo_gather', 'zmpo_scatter', 'zmpo_convert',
            'zmpo_scratch_bytes', 'zmpo_streamgemm_static', 'zmpo_alloc', 'zmpo_free',
-           'zmpo_memcpy', 'zmpo_seq_create', 'zmpo_seq_destroy', 'zmpo_seq_add',
+           'zmpo_memcpy', 'zmpo_syevd_sectors', 'zmpo_schedule_build', 'zmpo_seq_create', 'zmpo_seq_destroy', 'zmpo_seq_add',
            'zmpo_seq_length', 'zmpo_seq_run']
 
 
@@ -104,6 +104,8 @@ def lib():
     L.zmpo_alloc.argtypes = [ctypes.POINTER(vp), i64]
     L.zmpo_free.argtypes = [vp]
     L.zmpo_memcpy.argtypes = [vp, vp, i64, i32, vp]
+    L.zmpo_syevd_sectors.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.zmpo_schedule_build.argtypes = [i64, vp, vp, i32, i32, i64, vp, vp, i64, vp, i64, vp]
     L.zmpo_seq_create.argtypes = [ctypes.POINTER(vp)]
     L.zmpo_seq_destroy.argtypes = [vp]
     L.zmpo_seq_add.argtypes = [vp, vp]

@@ -37,6 +37,11 @@ TABLE_PATH = os.environ.get('ZMPO_TILE_TABLE',
                             os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                          'tiles_mi355x.json'))
 REPS = 2
+# host cost per schedule unit in ms per sigma application: building and
+# uploading a unit costs ~40 ns (more on freshly faulted memory) and a
+# program's schedule serves the ~20 sigma applications of one Davidson
+# solve, so each unit adds ~2 ns to every application
+HOST_MS_PER_UNIT = 2e-6
 
 # candidates per role: (kernel, qtensor.SHAPES id)
 CANDIDATES = {
@@ -47,6 +52,11 @@ CANDIDATES = {
     'sigma_r': [('stream', 2), ('shaped', 2), ('shaped', 3), ('shaped', 6), ('shaped', 8),
                 ('shaped', 10), ('shaped', 11)],
 }
+# the three launches of a renormalisation step (site x environment, MPO x
+# that, x site into the new environment)
+for _r in ('renorm1', 'renorm2', 'renorm3'):
+    CANDIDATES[_r] = [('shaped', 0), ('shaped', 1), ('shaped', 2), ('shaped', 3), ('shaped', 4),
+                      ('shaped', 5), ('shaped', 6), ('shaped', 7), ('shaped', 10), ('shaped', 11)]
 
 _table = None
 _online = {}
@@ -111,14 +121,18 @@ def lookup(sig, max_dist=None):
 def tune(prog, role, dtype, launch, reps=REPS):
     """Time every candidate of ``role`` for prog (``launch()`` runs prog in
     kernel dtype code ``dtype`` with its current per-dtype choice) and leave
-    the fastest set.  Returns ({(kernel, shape): ms}, best)."""
+    the one with the lowest kernel time plus host schedule cost
+    (HOST_MS_PER_UNIT per unit) set.  Returns ({(kernel, shape): ms}, best)."""
     torch = dv.torch()
     stream = torch.cuda.current_stream()
     times = {}
+    score = {}
     for kernel, shape in CANDIDATES[role]:
         prog.tiles[dtype] = (kernel, shape)
         launch()                      # builds and uploads the schedule
         dv.sync()
+        k_eff = prog.launch_choice(dtype)[0]
+        nunits = (prog._sched or {}).get((k_eff, shape), {}).get('nunits', 0)
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
@@ -127,7 +141,9 @@ def tune(prog, role, dtype, launch, reps=REPS):
         ev1.record(stream)
         ev1.synchronize()
         times[(kernel, shape)] = ev0.elapsed_time(ev1) / reps
-    best = min(times, key=times.get)
+        score[(kernel, shape)] = times[(kernel, shape)] + HOST_MS_PER_UNIT * nunits
+        prog.__dict__.setdefault('_tune_units', {})[(kernel, shape)] = nunits
+    best = min(score, key=score.get)
     # keep only the chosen schedule on the device
     if prog._sched:
         keep = prog.launch_choice(dtype, best[0])[0], best[1]
@@ -164,6 +180,7 @@ def main():
                     help='comma-separated K:D workloads')
     ap.add_argument('--dtypes', default='0,1,2')
     ap.add_argument('--budget-gb', type=float, default=96.0)
+    ap.add_argument('--roles', default='sigma,renorm')
     ap.add_argument('--out', default=TABLE_PATH)
     args = ap.parse_args()
     dv.require()
@@ -171,8 +188,9 @@ def main():
     log = []
     for kd in args.sizes.split(','):
         K, D = (int(x) for x in kd.split(':'))
+        extra = {}
         mpo, isite, st, L, R, prog = synthetic.middle_bond(
-            K, D, 0, int(args.budget_gb * 2 ** 30),
+            K, D, 0, int(args.budget_gb * 2 ** 30), extra=extra,
             log=lambda m: print('[autotune] ' + m, file=sys.stderr, flush=True))
         n = st.qtmp.size_allowed
         psi_h = np.random.default_rng(1).standard_normal(n)
@@ -197,7 +215,9 @@ def main():
                     entries[sig] = best
                     rec = {'K': K, 'D': D, 'signature': list(sig), 'kernel': best[0],
                            'shape': best[1],
-                           'ms': {'%s:%d' % kk: round(v, 3) for kk, v in times.items()}}
+                           'ms': {'%s:%d' % kk: round(v, 3) for kk, v in times.items()},
+                           'units': {'%s:%d' % kk: int(v) for kk, v in
+                                     getattr(pg, '_tune_units', {}).items()}}
                     log.append(rec)
                     print('[autotune] K=%d D=%d %s dtype %d -> %s (%.1fs) %s' % (
                         K, D, role, code, best, time.time() - t, json.dumps(rec['ms'])),
@@ -205,6 +225,40 @@ def main():
             prog._buffers.clear()
             for env in (L, R):
                 env.drop_low()
+            if dt == 0 and 'renorm' in args.roles:
+                # the renormalisation launches of the step that built L
+                from . import kernelQt as kq2
+                rp = kq2.RenormProgram(mpo, extra['lidx'], extra['lsite'], extra['Lprev'], 'L')
+                sv = extra['lsite'].value
+                sv = dv.upload(sv, 0) if isinstance(sv, np.ndarray) else sv
+                E = extra['Lprev'].buffer(0)
+                W = mpo.buffer(extra['lidx'], 0)
+                r1 = dv.empty(rp.inter[0], 0)
+                r2 = dv.empty(rp.inter[1], 0)
+                ro = dv.zeros(rp.out_size, 0)
+                from .qtensor import run_program as rpg
+                for st_ in rp.stages:
+                    for k, (pg, a, b, c) in enumerate(((st_[0], sv, E, r1), (st_[1], W, r1, r2),
+                                                       (st_[2], r2, sv, ro))):
+                        role = 'renorm%d' % (k + 1)
+                        if pg.ntasks == 0:
+                            continue
+                        sig = signature(pg, role, 0)
+                        if sig in entries:
+                            continue
+                        t = time.time()
+                        times, best = tune(pg, role, 0,
+                                           lambda pg=pg, a=a, b=b, c=c: rpg(pg, a, b, c, 0))
+                        entries[sig] = best
+                        rec = {'K': K, 'D': D, 'signature': list(sig), 'kernel': best[0],
+                               'shape': best[1],
+                               'ms': {'%s:%d' % kk: round(v, 3) for kk, v in times.items()},
+                               'units': {'%s:%d' % kk: int(v) for kk, v in
+                                         getattr(pg, '_tune_units', {}).items()}}
+                        log.append(rec)
+                        print('[autotune] K=%d D=%d %s -> %s (%.1fs) %s' % (
+                            K, D, role, best, time.time() - t, json.dumps(rec['ms'])), flush=True)
+                del r1, r2, ro, rp
         del mpo, L, R, prog
         dv.torch().cuda.empty_cache()
     out = {'device': 'MI355X (gfx950)',

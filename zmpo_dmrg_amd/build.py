@@ -10,7 +10,9 @@ OUT = os.path.join(HERE, 'libzmpo.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
-         '-I' + os.path.join(ROOT, 'include')]
+         '-I' + os.path.join(ROOT, 'include'),
+         # rocSOLVER: the per-sector eigensolver of the truncation
+         '-L/opt/rocm/lib', '-Wl,-rpath,/opt/rocm/lib', '-lrocsolver', '-lrocblas']
 
 
 def build(verbose=False):

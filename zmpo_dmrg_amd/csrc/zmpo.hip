@@ -10,6 +10,7 @@
 // LDS-staged loads instead of being materialised by transpose copies.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -17,6 +18,8 @@
 #include <mutex>
 #include <string>
 #include <vector>
+
+#include <rocsolver/rocsolver.h>
 
 #include "../../include/zmpo.h"
 
@@ -2520,6 +2523,157 @@ int zmpo_convert(int dtype_dst, void* dst, int dtype_src, const void* src, int64
   ZMPO_CONV(ZMPO_F32, float, ZMPO_BF16, bf16s)
 #undef ZMPO_CONV
   return set_error(ZMPO_ERR_ARG, "convert: unsupported dtype pair");
+}
+
+// ------------------------------------------------------- schedule builder
+// Host-side construction of the unit schedule of a grouped-GEMM launch (the
+// work items of zmpo_wavegemm / zmpo_shapedgemm / zmpo_streamgemm): one
+// unit per (output tile, chunk of the task's pairs), chunks cut where the
+// summed K of the preceding pairs of the task crosses a multiple of kchunk.
+// Unit order: (task, tile, chunk), or (key, chunk, task, tile) when a reuse
+// key per task is given; split-K slots are numbered in (task, tile, chunk)
+// order over the tasks with more than one chunk; one reduction group per
+// split tile, in unit order.  tasks / pairs / reuse_key are host arrays.
+int zmpo_schedule_build(int64_t nt, const zmpo_task* tasks, const zmpo_pair* pairs, int tm,
+                        int tn, int64_t kchunk, const int64_t* reuse_key, zmpo_unit* units,
+                        int64_t units_cap, zmpo_rgroup* groups, int64_t groups_cap,
+                        int64_t* counts) {
+  if (nt < 0 || tm <= 0 || tn <= 0 || kchunk <= 0 || !counts || (nt > 0 && !tasks))
+    return set_error(ZMPO_ERR_ARG, "schedule_build: bad arguments");
+  std::vector<int64_t> nch(nt), cfirst(nt + 1, 0), tiles(nt);
+  std::vector<int32_t> cb, ce;
+  cb.reserve(nt);
+  ce.reserve(nt);
+  for (int64_t t = 0; t < nt; ++t) {
+    const zmpo_task& tk = tasks[t];
+    const int64_t rows = tk.M > 0 ? ((int64_t)tk.M + tm - 1) / tm : 1;
+    const int64_t cols = tk.N > 0 ? ((int64_t)tk.N + tn - 1) / tn : 1;
+    tiles[t] = (rows > 0 ? rows : 1) * (cols > 0 ? cols : 1);
+    cfirst[t] = (int64_t)cb.size();
+    if (tk.p_end <= tk.p_begin) {
+      cb.push_back(tk.p_begin);
+      ce.push_back(tk.p_begin);
+    } else {
+      int64_t pre = 0, cur = -1;
+      for (int32_t q = tk.p_begin; q < tk.p_end; ++q) {
+        const int64_t cid = pre / kchunk;
+        if (cid != cur) {
+          cb.push_back(q);
+          ce.push_back(q + 1);
+          cur = cid;
+        } else {
+          ce.back() = q + 1;
+        }
+        pre += pairs[q].K;
+      }
+    }
+    nch[t] = (int64_t)cb.size() - cfirst[t];
+  }
+  // split-K slot base per task ((task, tile, chunk) order)
+  std::vector<int64_t> sbase(nt, -1);
+  int64_t nslots = 0, nunits = 0, ngroups = 0;
+  for (int64_t t = 0; t < nt; ++t) {
+    nunits += tiles[t] * nch[t];
+    if (nch[t] > 1) {
+      sbase[t] = nslots;
+      nslots += tiles[t] * nch[t];
+      ngroups += tiles[t];
+    }
+  }
+  counts[0] = nunits;
+  counts[1] = ngroups;
+  counts[2] = nslots;
+  if (!units) return ZMPO_OK;  // count only
+  if (units_cap < nunits || (ngroups > 0 && (!groups || groups_cap < ngroups)))
+    return set_error(ZMPO_ERR_ARG, "schedule_build: output arrays too small");
+  int64_t u = 0, gidx = 0;
+  auto emit = [&](int64_t t, int64_t ti, int64_t c) {
+    const int64_t cols = tiles[t] / ((int64_t)tasks[t].M > 0 ? ((int64_t)tasks[t].M + tm - 1) / tm : 1);
+    zmpo_unit& un = units[u++];
+    un.task = (int32_t)t;
+    un.m0 = (int32_t)((ti / cols) * tm);
+    un.n0 = (int32_t)((ti % cols) * tn);
+    un.p_begin = cb[cfirst[t] + c];
+    un.p_end = ce[cfirst[t] + c];
+    un.slot = nch[t] > 1 ? (int32_t)(sbase[t] + ti * nch[t] + c) : -1;
+    un.pad0 = un.pad1 = 0;
+    if (nch[t] > 1 && c == 0) {
+      zmpo_rgroup& g = groups[gidx++];
+      g.task = (int32_t)t;
+      g.m0 = un.m0;
+      g.n0 = un.n0;
+      g.slot_begin = un.slot;
+      g.nslots = (int32_t)nch[t];
+      g.pad = 0;
+    }
+  };
+  if (!reuse_key) {
+    for (int64_t t = 0; t < nt; ++t)
+      for (int64_t ti = 0; ti < tiles[t]; ++ti)
+        for (int64_t c = 0; c < nch[t]; ++c) emit(t, ti, c);
+  } else {
+    std::vector<int64_t> ord(nt);
+    for (int64_t t = 0; t < nt; ++t) ord[t] = t;
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int64_t a, int64_t b) { return reuse_key[a] < reuse_key[b]; });
+    for (int64_t i = 0; i < nt;) {
+      int64_t j = i, cmax = 0;
+      while (j < nt && reuse_key[ord[j]] == reuse_key[ord[i]]) {
+        cmax = nch[ord[j]] > cmax ? nch[ord[j]] : cmax;
+        ++j;
+      }
+      for (int64_t c = 0; c < cmax; ++c)
+        for (int64_t k = i; k < j; ++k) {
+          const int64_t t = ord[k];
+          if (c >= nch[t]) continue;
+          for (int64_t ti = 0; ti < tiles[t]; ++ti) emit(t, ti, c);
+        }
+      i = j;
+    }
+  }
+  if (u != nunits || gidx != ngroups) return set_error(ZMPO_ERR_ARG, "schedule_build: count mismatch");
+  return ZMPO_OK;
+}
+
+// ------------------------------------------------- truncation eigensolver
+// Eigenvalues (ascending) and eigenvectors of nsec symmetric sector
+// matrices packed in A (sector s: n_s x n_s at offset off_s, any storage
+// order since it is symmetric), by rocSOLVER's divide-and-conquer dsyevd on
+// the caller's stream; eigenvector j of sector s overwrites
+// A[off_s + j*n_s .. off_s + (j+1)*n_s).  W gets the eigenvalues at woff_s.
+// E (>= max n_s doubles) and info (>= nsec ints) are device workspace; a
+// nonzero info[s] after the stream completes means no convergence.
+int zmpo_syevd_sectors(int nsec, const int64_t* n, const int64_t* off, const int64_t* woff,
+                       double* A, double* W, double* E, int* info, void* stream) {
+  if (nsec < 0) return set_error(ZMPO_ERR_ARG, "syevd_sectors: negative sector count");
+  if (nsec == 0) return ZMPO_OK;
+  if (!n || !off || !woff || !A || !W || !E || !info)
+    return set_error(ZMPO_ERR_ARG, "syevd_sectors: null pointer");
+  constexpr int MAXDEV = 64;
+  static rocblas_handle handles[MAXDEV] = {nullptr};
+  static std::mutex mu;
+  int dev = 0;
+  int rc = check_hip(hipGetDevice(&dev), "syevd_sectors: device");
+  if (rc) return rc;
+  if (dev < 0 || dev >= MAXDEV) return set_error(ZMPO_ERR_ARG, "syevd_sectors: device index beyond 63");
+  rocblas_handle h = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (!handles[dev] && rocblas_create_handle(&handles[dev]) != rocblas_status_success)
+      return set_error(ZMPO_ERR_HIP, "syevd_sectors: rocblas_create_handle failed");
+    h = handles[dev];
+  }
+  if (rocblas_set_stream(h, (hipStream_t)stream) != rocblas_status_success)
+    return set_error(ZMPO_ERR_HIP, "syevd_sectors: rocblas_set_stream failed");
+  for (int s = 0; s < nsec; ++s) {
+    if (n[s] <= 0) continue;
+    if (n[s] > 0x7fffffffLL) return set_error(ZMPO_ERR_ARG, "syevd_sectors: sector too large");
+    const rocblas_status st =
+        rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n[s],
+                         A + off[s], (rocblas_int)n[s], W + woff[s], E, info + s);
+    if (st != rocblas_status_success) return set_error(ZMPO_ERR_HIP, "syevd_sectors: rocsolver_dsyevd failed");
+  }
+  return check_hip(hipGetLastError(), "syevd_sectors");
 }
 
 // ------------------------------------------------------------ device store

@@ -14,6 +14,7 @@ import time
 
 import numpy as np
 
+from . import device as dv
 from . import kernelQt
 from .dot import ci_solver
 from .mpo import MPOStore, qphys
@@ -121,15 +122,22 @@ def restart_from_left_mps(dmrg, sites, qnuml):
 
 def sweep_updateSite(dmrg, isite, qred, site, status):
     """opt.py:313-344 (ncsite=2)."""
+    # site: dense (host decimation) or the QTensor the device decimation built
     if status == 'L':
         dmrg.qnuml[isite + 1] = qred.copy()
-        q = QTensor([False, False, True])
-        q.fromDenseTensor(site, [dmrg.qlst[0], dmrg.qlst[1], qred])
+        if isinstance(site, QTensor):
+            q = site
+        else:
+            q = QTensor([False, False, True])
+            q.fromDenseTensor(site, [dmrg.qlst[0], dmrg.qlst[1], qred])
         dmrg.lmps[isite] = q
     else:
         dmrg.qnumr[isite + 1] = qred.copy()
-        q = QTensor([True, False, False])
-        q.fromDenseTensor(site, [qred, dmrg.qlst[-2], dmrg.qlst[-1]])
+        if isinstance(site, QTensor):
+            q = site
+        else:
+            q = QTensor([True, False, False])
+            q.fromDenseTensor(site, [qred, dmrg.qlst[-2], dmrg.qlst[-1]])
         dmrg.rmps[isite + 1] = q
     return q
 
@@ -137,7 +145,8 @@ def sweep_updateSite(dmrg, isite, qred, site, status):
 def sweep_updateBoundary(dmrg, qred, srotR, status):
     """opt.py:347-377 (ncsite=2)."""
     K = dmrg.nsite
-    smat = srotR[0] / np.linalg.norm(srotR[0])
+    s0 = srotR[0] if isinstance(srotR, np.ndarray) else dv.download(srotR[0])
+    smat = s0 / np.linalg.norm(s0)
     key = list(dmrg.qsectors.keys())[0]
     qsym = np.array([key])
     if status == 'L':

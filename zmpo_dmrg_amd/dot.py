@@ -167,7 +167,10 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
             dmrg.sigma_flops += prog.flops
 
         def hvec_tier(dt):
-            """Sigma with operands in dt (FP64 vectors in and out)."""
+            """Sigma with operands in dt (FP64 vectors in and out); None:
+            a callable that releases the intermediates of a finished tier."""
+            if dt is None:
+                return prog.drop_buffers
             if dt == dtype:
                 return hvec
 
@@ -185,8 +188,16 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
         solver.HVec = hvec
         v0 = None
         if dmrg.psi0 is not None:
-            v0 = np.ascontiguousarray(dmrg.psi0[:neig][:, prjmap])
-        if dtype and neig == 1 and PRECISION_MODE == 'correction':
+            if isinstance(dmrg.psi0, np.ndarray):
+                v0 = np.ascontiguousarray(dmrg.psi0[:neig][:, prjmap])
+            else:
+                # guess left on the device by the previous step's decimation
+                from .qtensor import gather
+                idx = dv.upload(np.asarray(prjmap, dtype=np.int64))
+                v0 = dv.empty(neig * ndim0, 0).view(neig, ndim0)
+                for e in range(neig):
+                    gather(v0[e], dmrg.psi0[e], idx)
+        if dtype and neig == 1 and PRECISION_MODE in ('correction', 'rq'):
             eig, vec, nmvp = _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, ndim0,
                                                 dtype)
             for env in (fL, fR):
@@ -203,7 +214,7 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
             eig, vec, nmvp = solver.solve_iter(v0)
         else:
             eig, vec, nmvp = solver.solve_iter(v0)
-        if dtype and not (neig == 1 and PRECISION_MODE == 'correction'):
+        if dtype and not (neig == 1 and PRECISION_MODE in ('correction', 'rq')):
             # the low-precision copies of this step's environments are not
             # needed again (the next step at this bond rebuilds them): drop
             # them so that only the FP64 environments stay resident
@@ -220,15 +231,37 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
         # the sigma intermediates are not needed before the next step's
         # program: free them ahead of the decimation's own buffers
         prog._buffers.clear()
+        eigs += list(eig)
+        nmvps += nmvp
+        dmrg.niters.append(solver.niter)
+        if _device_decimation(dmrg):
+            dvec = (vec, prjmap)
+            continue
         vec_h = dv.download(vec)
         civec = np.zeros((neig, dmrg.dims[3]))
         civec[:, prjmap] = vec_h
         civecs.append(civec)
-        eigs += list(eig)
-        nmvps += nmvp
-        dmrg.niters.append(solver.niter)
-    civecs = np.vstack(civecs)
     t0 = time.time()
+    if _device_decimation(dmrg):
+        from .decim import DeviceDecimation
+        dd = DeviceDecimation(dmrg, dvec[0], dvec[1], isite, status)
+        if status == 'L':
+            classes = dpt(dmrg.qnuml[isite], dmrg.qphys[isite])
+        else:
+            classes = dpt(dmrg.qphys[isite + 1], dmrg.qnumr[isite + 2])
+        if dd.run(classes, dmrg.thresh, dmrg.Dmax):
+            site, srotR = dd.site_and_srotR(dmrg.qlst)
+            t0 = lap('decimation', t0)
+            if isite == actlst[-1]:
+                dmrg.psi0 = dd.C.view(dd.neig, -1)
+            else:
+                nxt = dmrg.lmps[isite - 1] if status == 'R' else dmrg.rmps[isite + 2]
+                dmrg.psi0 = dd.guess(nxt)
+            lap('guess', t0)
+            return nmvps, np.array(eigs), dd.sigs, dd.dwts, dd.qred, site, srotR
+        # an all-zero density matrix: the host path handles it
+        civecs = [dv.download(dd.C).reshape(dd.neig, -1)]
+    civecs = np.vstack(civecs)
     sigs, dwts, qred, site, srotR = decimation(dmrg, civecs, isite, status, info)
     t0 = lap('decimation', t0)
     genGuess(dmrg, isite, actlst, civecs, srotR, status)
@@ -236,13 +269,26 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
     return nmvps, np.array(eigs), sigs, dwts, qred, site, srotR
 
 
-# Precision form (dmrg.dtype 1 = FP32, 2 = BF16 sigma operands):
-# 'correction' (default): low-precision Davidson phases down to the level
-# their rounding allows, then outer FP64 corrections around anchored
-# low-precision inner solves (_defect_correction); 'restart': a
-# low-precision Davidson to residual 1e-4 followed by a full FP64 Davidson
-# from its vector.
-PRECISION_MODE = os.environ.get('ZMPO_PRECISION_MODE', 'correction')
+# Decimation and guess on the device (decim.py) unless ZMPO_DECIM=host;
+# the noise term and several symmetry sectors take the host path.
+DECIM = os.environ.get('ZMPO_DECIM', 'device')
+
+
+def _device_decimation(dmrg):
+    return (DECIM == 'device' and len(dmrg.qsectors) == 1
+            and float(getattr(dmrg, 'noise', 0.0)) < 1e-10
+            and dv.device().type == 'cuda')
+
+
+# Precision form (dmrg.dtype 1 = FP32, 2 = BF16 sigma operands), all in
+# _defect_correction except 'restart':
+# 'rq' (default): low-precision Davidson phases (the reference's iteration
+# budget), then one FP64 sigma whose Rayleigh quotient is the reported
+# eigenvalue; 'correction': the same phases, then outer FP64 corrections
+# around anchored low-precision inner solves until the FP64 residual meets
+# the Davidson criteria; 'restart': a low-precision Davidson to residual
+# 1e-4 followed by a full FP64 Davidson from its vector.
+PRECISION_MODE = os.environ.get('ZMPO_PRECISION_MODE', 'rq')
 # inner solves stop when their residual falls below this fraction of the
 # current FP64 residual, or after this many iterations
 INNER_FRAC = 0.05
@@ -254,7 +300,8 @@ OUTER_MAX = 8
 # (BF16 operands) and 2.2e-7 (FP32) at K = 36, D = 1000
 TIERS = {1: (1,), 2: (2, 1)}
 TIER_TOL = {2: 4e-3, 1: 1e-6}
-TIER_MAXCYCLE = 40
+# iterations of a BF16 phase (the last tier gets the solver's own budget)
+TIER_MAXCYCLE = {2: 12}
 TIER_HERM = {2: 2e-2, 1: 1e-5}
 
 
@@ -266,8 +313,12 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
     FP32; FP32 runs: FP32), one Davidson with that tier's sigma -- one
     search space, from the reference's start vector or the previous tier's
     vector -- until its residual reaches what the tier's rounding resolves
-    (TIER_TOL x max(1, |theta|)).  Then outer FP64 iterations from that
-    vector: y = H x_k with FP64 sigma, theta_k = x_k.y (x_k normalised),
+    (TIER_TOL x max(1, |theta|)) or the iteration budget ends (the last
+    tier has the solver's own, as the reference's Davidson).  Mode 'rq'
+    then returns the FP64 Rayleigh quotient theta = x.Hx of the phases'
+    vector (one FP64 sigma; its error is of the order of |r|^2 / gap for
+    the FP64 residual r).  Mode 'correction' goes on with outer FP64
+    iterations from that vector: y = H x_k with FP64 sigma, theta_k = x_k.y (x_k normalised),
     r_k = y - theta_k x_k.  Stop when |r_k| <= crit_vec or |theta_k -
     theta_(k-1)| <= crit_e (the Davidson criteria of pdvdson.py:161-188, on
     FP64 quantities).  Otherwise an inner Davidson with the most precise
@@ -287,8 +338,10 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
         d = dv.download(solver.diag) + 1e-12 * np.arange(1, 1 + n) / float(n)
         x = t.zeros(n, dtype=t.float64, device=dv.device())
         x[int(np.argsort(d, kind='stable')[0])] = 1.0
-    else:
+    elif isinstance(v0, np.ndarray):
         x = dv.upload(np.asarray(v0).reshape(-1), 0)
+    else:
+        x = v0.reshape(-1).clone()
     x.mul_(1.0 / ops.norm(x))
     y = t.empty_like(x)
     r = t.empty_like(x)
@@ -298,7 +351,8 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
     # ---- low-precision phases
     for tier in tiers:
         ph = EigenSolver()
-        ph.maxcycle = TIER_MAXCYCLE
+        ph.maxcycle = TIER_MAXCYCLE.get(tier, solver.maxcycle) if tier != tiers[-1] \
+            else solver.maxcycle
         ph.crit_e = -1.0                  # residual criterion only
         ph.ndim, ph.neig, ph.diag = n, 1, solver.diag
         ph.herm_tol = TIER_HERM[tier]
@@ -317,9 +371,23 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
         niter += ph.niter
         x.copy_(vec[0])
         x.mul_(1.0 / ops.norm(x))
+    # only the last tier runs again (inner solves, 'correction' mode):
+    # release the others' intermediates before the FP64 ones are allocated
+    for tier in (tiers if PRECISION_MODE == 'rq' else tiers[:-1]):
+        hvec_tier(None)(tier)
     hvec_lo = hvec_tier(tiers[-1])
     herm = TIER_HERM[tiers[-1]]
     theta_prev = rn_prev = None
+    if PRECISION_MODE == 'rq':
+        # the reported eigenvalue: FP64 Rayleigh quotient of the phases' vector
+        hvec64(x, y)
+        nmvp += 1
+        theta = float(ops.dots(x.view(1, -1), 1, y)[0])
+        r.copy_(y)
+        r.add_(x, alpha=-theta)
+        dmrg.last_residual64 = ops.norm(r)
+        solver.niter = niter
+        return np.array([theta]), x.view(1, -1), nmvp
 
     def fallback():
         fine = EigenSolver()

@@ -365,6 +365,16 @@ class SigmaProgram:
             if ch is not None:
                 ch._dev = None
 
+    def drop_buffers(self, dtype):
+        """Release the intermediates of one operand type (and the split-K
+        workspace of its launches); the next apply in that type allocates
+        them again."""
+        from .qtensor import _WORK
+        self._buffers.pop(dtype, None)
+        self._buffers.pop(('t1b', dtype), None)
+        for code in {dtype, out_dtype(dtype)}:
+            _WORK.pop(code, None)
+
     def buffers(self, dtype, mid=None):
         """Intermediate buffers [top1, top2, top3]; top2 is allocated only for
         the unfused path (mid=True, default: not self.fused)."""
@@ -760,6 +770,14 @@ class RenormProgram:
         out = dv.zeros(self.out_size, dtype)
         W = mpo.buffer(self.site_idx, dtype)
         E = env_in.buffer(dtype)
+        if getattr(out, 'is_cuda', False):
+            # tiles of the three launches (autotune.py: the measured table,
+            # or tuning on the device in 'online' mode)
+            for st in self.stages:
+                for k, (pg, a, b, c) in enumerate(((st[0], site_values, E, t1),
+                                                   (st[1], W, t1, t2), (st[2], t2, site_values, out))):
+                    autotune.choose(pg, 'renorm%d' % (k + 1), dtype,
+                                    lambda pg=pg, a=a, b=b, c=c: run_program(pg, a, b, c, dtype))
         sq = self.sequence(dtype) if SIGMA_SEQ and getattr(out, 'is_cuda', False) else None
         if sq is not None:
             seq, nwork = sq

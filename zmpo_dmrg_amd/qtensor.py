@@ -463,6 +463,31 @@ def finish_tasks(tasks):
 
 
 def make_schedule(tasks, pairs, rb, kchunk=2048, reuse_key=None, tile=None):
+    """Units (one output tile x one pair chunk) and split-K groups, built
+    by the library's host-side schedule builder (zmpo_schedule_build; the
+    same units, order, slots and groups as make_schedule_np, which states
+    them in numpy).  Returns (units, groups, number of split-K slots)."""
+    TMs, TNs = (16 * rb, 16) if tile is None else tile
+    t = np.ascontiguousarray(tasks, dtype=_lib.TASK_DTYPE)
+    p = np.ascontiguousarray(pairs, dtype=_lib.PAIR_DTYPE)
+    key = None if reuse_key is None else np.ascontiguousarray(reuse_key, dtype=np.int64)
+    if key is not None and len(key) != len(t):
+        raise ValueError('reuse_key needs one value per task')
+    L = _lib.lib()
+    counts = np.zeros(3, np.int64)
+    kp = key.ctypes.data if key is not None else None
+    _lib.check(L.zmpo_schedule_build(len(t), t.ctypes.data, p.ctypes.data, TMs, TNs, int(kchunk),
+                                     kp, None, 0, None, 0, counts.ctypes.data))
+    units = np.zeros(int(counts[0]), dtype=_lib.UNIT_DTYPE)
+    groups = np.zeros(int(counts[1]), dtype=_lib.RGROUP_DTYPE)
+    _lib.check(L.zmpo_schedule_build(len(t), t.ctypes.data, p.ctypes.data, TMs, TNs, int(kchunk),
+                                     kp, units.ctypes.data, len(units),
+                                     groups.ctypes.data if len(groups) else None, len(groups),
+                                     counts.ctypes.data))
+    return units, groups, int(counts[2])
+
+
+def make_schedule_np(tasks, pairs, rb, kchunk=2048, reuse_key=None, tile=None):
     """Units (one output tile x one pair chunk) and split-K groups.
 
     The tile is (16*rb) x 16 for the wave kernel, or tile=(TM, TN) for the
