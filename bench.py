@@ -279,6 +279,17 @@ def main():
     # launch issues one dispatch per LDS size class), so the per-dispatch
     # average can be set beside rocprofv3's kernel statistics
     ndisp = sum(len(ch.launches) for ch in prog.chains) if (prog.fused and dom == 1) else nlaunch
+    # algorithmic bytes of one sigma by the reference-order rule: every
+    # operator's L_p, R_p, W_p[i], W_p[i+1] read once, psi read and sigma
+    # written once (intermediates excluded; they are reported beside it)
+    env_el = sum(int(m.size_allowed) for m in L.meta) + sum(int(m.size_allowed) for m in R.meta)
+    w_el = sum(int(mpo.sites[s_][p].size_allowed) for s_ in (isite, isite + 1)
+               for p in range(mpo.nops))
+    sigma_bytes = 8 * (env_el + w_el + 2 * int(st.qtmp.size_allowed))
+    # top1 and top3 written once and read once per sigma (FP64 elements)
+    inter_bytes = 8 * 2 * sum(
+        int(pg.tasks_host['M'].astype(np.int64) @ pg.tasks_host['N'].astype(np.int64))
+        for stg in prog.stages for pg in (stg[0], stg[2]) if pg.ntasks)
     roof = {'kernel': '%s, sigma launch %s' % (kname, names[dom]),
             'bound': bound,
             'achieved': ach_b / 1e9 if bound == 'hbm' else ach_f / 1e12,
@@ -291,6 +302,9 @@ def main():
             'achieved_tflops': ach_f / 1e12, 'mfma_frac': ach_f / PEAK[dtype],
             'launch_ms': avg_ms, 'launches_per_step': nlaunch,
             'dispatches_per_step': ndisp, 'dispatch_avg_ms': stage_ms[dom] / ndisp,
+            'sigma_alg_bytes': sigma_bytes,
+            'sigma_hbm_floor_ms': sigma_bytes / HBM * 1e3,
+            'intermediate_bytes': inter_bytes,
             'stage_names': names,
             'stage_ms': [round(x, 4) for x in stage_ms],
             'stage_gflop': [round(x / 1e9, 3) for x in stage_fl],

@@ -59,6 +59,38 @@ def zeros(n, code=0):
     return torch().zeros(int(max(n, 1)), dtype=tdtype(code), device=device())
 
 
+ELSIZE = {0: 8, 1: 4, 2: 2, 3: 4}
+_SCRATCH = {}
+
+
+def scratch(name, nbytes):
+    """A device byte buffer of at least nbytes, one per name, reused by
+    every caller (grown when too small, never shrunk): the sigma and
+    renormalisation intermediates of all operand types share one, so the
+    precision tiers and successive steps do not reallocate them."""
+    require()
+    key = (str(device()), name)
+    buf = _SCRATCH.get(key)
+    if buf is None or buf.numel() < nbytes:
+        _SCRATCH.pop(key, None)        # release the old buffer first
+        buf = None
+        buf = torch().empty(int(max(nbytes, 256)), dtype=torch().uint8, device=device())
+        _SCRATCH[key] = buf
+    return buf
+
+
+def typed(buf, off_bytes, n, code):
+    """n elements of dtype code viewed at byte offset off_bytes (a multiple
+    of 8) of a scratch buffer."""
+    es = ELSIZE[code]
+    return buf[off_bytes:off_bytes + int(max(n, 1)) * es].view(tdtype(code))
+
+
+def release_scratch():
+    """Drop every scratch buffer (the memory returns to the allocator)."""
+    _SCRATCH.clear()
+
+
 def upload(arr, code=None):
     """Host numpy -> device tensor (float64/float32/int64 by dtype)."""
     require()

@@ -348,8 +348,14 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
     nmvp = 0
     niter = 0
     tiers = TIERS[dtype]
+    tm = dmrg.__dict__.setdefault('timing', {})
+
+    def lap(key, t0):
+        dv.sync()
+        tm[key] = tm.get(key, 0.0) + time.time() - t0
     # ---- low-precision phases
     for tier in tiers:
+        t0 = time.time()
         ph = EigenSolver()
         ph.maxcycle = TIER_MAXCYCLE.get(tier, solver.maxcycle) if tier != tiers[-1] \
             else solver.maxcycle
@@ -371,6 +377,7 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
         niter += ph.niter
         x.copy_(vec[0])
         x.mul_(1.0 / ops.norm(x))
+        lap('tier%d' % tier, t0)
     # only the last tier runs again (inner solves, 'correction' mode):
     # release the others' intermediates before the FP64 ones are allocated
     for tier in (tiers if PRECISION_MODE == 'rq' else tiers[:-1]):
@@ -380,12 +387,14 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
     theta_prev = rn_prev = None
     if PRECISION_MODE == 'rq':
         # the reported eigenvalue: FP64 Rayleigh quotient of the phases' vector
+        t0 = time.time()
         hvec64(x, y)
         nmvp += 1
         theta = float(ops.dots(x.view(1, -1), 1, y)[0])
         r.copy_(y)
         r.add_(x, alpha=-theta)
         dmrg.last_residual64 = ops.norm(r)
+        lap('rq64', t0)
         solver.niter = niter
         return np.array([theta]), x.view(1, -1), nmvp
 

@@ -366,25 +366,34 @@ class SigmaProgram:
                 ch._dev = None
 
     def drop_buffers(self, dtype):
-        """Release the intermediates of one operand type (and the split-K
-        workspace of its launches); the next apply in that type allocates
-        them again."""
-        from .qtensor import _WORK
+        """Forget the intermediate views of one operand type (their memory is
+        the shared scratch buffer, kept for the next launches)."""
         self._buffers.pop(dtype, None)
         self._buffers.pop(('t1b', dtype), None)
-        for code in {dtype, out_dtype(dtype)}:
-            _WORK.pop(code, None)
+
+    def _layout(self, mid):
+        """Byte offsets of top1, the second top1 buffer, top3 and top2 in the
+        shared intermediate buffer, sized for FP64 so that every operand
+        type fits the same bytes."""
+        seg = [((n * 8 + 255) // 256) * 256 for n in self.inter]
+        off = [0, seg[0], 2 * seg[0], 2 * seg[0] + seg[2]]
+        return off, off[3] + (seg[1] if mid else 0)
 
     def buffers(self, dtype, mid=None):
-        """Intermediate buffers [top1, top2, top3]; top2 is allocated only for
-        the unfused path (mid=True, default: not self.fused)."""
+        """Intermediate buffers [top1, top2, top3] (views of the scratch
+        buffer that the sigma and renormalisation launches of every
+        operand type share); top2 only for the unfused path (mid=True,
+        default: not self.fused)."""
         mid = (not self.fused) if mid is None else mid
-        if dtype not in self._buffers:
-            self._buffers[dtype] = [dv.empty(self.inter[0], dtype), None,
-                                    dv.empty(self.inter[2], dtype)]
-        b = self._buffers[dtype]
+        off, total = self._layout(True)
+        buf = dv.scratch('inter', total if mid else off[3])
+        b = self._buffers.get(dtype)
+        if b is None or b[0].data_ptr() != buf[0:1].data_ptr():
+            b = [dv.typed(buf, off[0], self.inter[0], dtype), None,
+                 dv.typed(buf, off[2], self.inter[2], dtype)]
+            self._buffers[dtype] = b
         if mid and b[1] is None:
-            b[1] = dv.empty(self.inter[1], dtype)
+            b[1] = dv.typed(buf, off[3], self.inter[1], dtype)
         return b
 
     def apply(self, mpo, lenv, renv, psi, sigma, dtype=0, fused=None, overlap=None):
@@ -456,10 +465,8 @@ class SigmaProgram:
         if getattr(self, '_side', None) is None:
             self._side = torch.cuda.Stream()   # default priority
         side = self._side
-        key = ('t1b', dtype)
-        if key not in self._buffers:
-            self._buffers[key] = dv.empty(self.inter[0], dtype)
-        bufs = [t1, self._buffers[key]]
+        off, _ = self._layout(False)
+        bufs = [t1, dv.typed(dv.scratch('inter', off[3]), off[1], self.inter[0], dtype)]
         free = [None, None]
         Pi, Pj = mpo.padded_buffer(self.isite, dtype), mpo.padded_buffer(self.isite + 1, dtype)
         side.wait_stream(main)   # psi and the buffers are ready
@@ -511,8 +518,7 @@ class SigmaGraph:
         # environments and the workspace are allocated outside the capture
         if warmup:
             prog.apply(mpo, lenv, renv, self.x, self.y, dtype, overlap=False)
-        from .qtensor import _WORK
-        self._keep = [lenv.buffer(dtype), renv.buffer(dtype), _WORK.get(dtype),
+        self._keep = [lenv.buffer(dtype), renv.buffer(dtype), dv._SCRATCH.get((str(dv.device()), 'splitk')),
                       mpo.buffer(prog.isite, dtype), mpo.buffer(prog.isite + 1, dtype)]
         if prog.fused:
             self._keep += [mpo.padded_buffer(prog.isite, dtype),
@@ -765,8 +771,11 @@ class RenormProgram:
 
     def apply(self, mpo, site_values, env_in, dtype=0):
         """Return the new EnvStore (values on the device)."""
-        t1 = dv.empty(self.inter[0], dtype)
-        t2 = dv.empty(self.inter[1], dtype)
+        # intermediates in the scratch buffer the sigma launches also use
+        s1 = ((self.inter[0] * 8 + 255) // 256) * 256
+        buf = dv.scratch('inter', s1 + self.inter[1] * 8)
+        t1 = dv.typed(buf, 0, self.inter[0], dtype)
+        t2 = dv.typed(buf, s1, self.inter[1], dtype)
         out = dv.zeros(self.out_size, dtype)
         W = mpo.buffer(self.site_idx, dtype)
         E = env_in.buffer(dtype)

@@ -5,6 +5,13 @@
         WRITE_SIZE counters: run it under ``rocprofv3 --pmc FETCH_SIZE`` (then
         WRITE_SIZE) and divide the known bytes by what the counter reports.
 
+    python -m zmpo_dmrg_amd.profiling stats KERNEL_STATS_CSV
+        markdown table of a rocprofv3 kernel statistics file.
+
+    python -m zmpo_dmrg_amd.profiling mfma COUNTER_CSV TRACE_CSV [--cu 256]
+        executed MFMA TFLOP/s and MFMA busy fraction per kernel from
+        SQ_INSTS_VALU_MFMA_MOPS_* / SQ_VALU_MFMA_BUSY_CYCLES / GRBM_GUI_ACTIVE.
+
     python -m zmpo_dmrg_amd.profiling launches BENCH_JSON TRACE_CSV
             [--fetch CSV] [--write CSV] [--fetch-scale F] [--write-scale W]
         per-launch duration and HBM bytes of the sigma launches (L*psi, the
@@ -20,7 +27,9 @@ import json
 import sys
 
 KERNELS = ('shaped_gemm', 'shaped_reduce', 'chain_gemm', 'stream_gemm', 'wave_gemm', 'wave_reduce',
-           'skinny_gemm', 'dot_gemm', 'blockgemm')
+           'skinny_gemm', 'dot_gemm', 'blockgemm', 'panel_gemm')
+# executed FLOPs per SQ_INSTS_VALU_MFMA_MOPS_* count
+FLOPS_PER_MOP = 512
 
 
 def _copy(gib):
@@ -108,6 +117,60 @@ def _launches(args):
     print(json.dumps({'launches': res}))
 
 
+def _short(name):
+    n = name.replace('void ', '').replace('(anonymous namespace)::', '')
+    return n.split('(zmpo')[0].split('(int,')[0].split('(float*')[0].split('(double*')[0]
+
+
+def _stats(args):
+    """Markdown table of a rocprofv3 ``*_kernel_stats.csv``."""
+    with open(args.csv) as f:
+        rows = list(csv.DictReader(f))
+    print('| kernel | calls | total ms | avg ms | min ms | max ms | % |')
+    print('|---|---|---|---|---|---|---|')
+    for r in rows[:args.top]:
+        print('| `%s` | %s | %.2f | %.3f | %.3f | %.3f | %.2f |' % (
+            _short(r['Name']), r['Calls'], float(r['TotalDurationNs']) / 1e6,
+            float(r['AverageNs']) / 1e6, float(r['MinNs']) / 1e6, float(r['MaxNs']) / 1e6,
+            float(r['Percentage'])))
+
+
+def _mfma(args):
+    """Per-kernel MFMA work from a counter-collection CSV with
+    SQ_INSTS_VALU_MFMA_MOPS_F64 (and _F32 / _BF16 when present),
+    SQ_VALU_MFMA_BUSY_CYCLES and GRBM_GUI_ACTIVE, joined with the kernel
+    durations of a kernel-trace CSV of the same command (dispatches in the
+    same order).  Executed TFLOP/s = FLOPS_PER_MOP * MOPS / duration; busy =
+    SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE * cu * 4 SIMDs)."""
+    per = collections.OrderedDict()
+    with open(args.counters) as f:
+        for r in csv.DictReader(f):
+            d = per.setdefault(r['Dispatch_Id'], {'name': _short(r['Kernel_Name'])})
+            d[r['Counter_Name']] = d.get(r['Counter_Name'], 0.0) + float(r['Counter_Value'])
+    dur = {}
+    with open(args.trace) as f:
+        for r in csv.DictReader(f):
+            dur.setdefault(_short(r['Kernel_Name']), []).append(
+                (int(r['End_Timestamp']) - int(r['Start_Timestamp'])) * 1e-9)
+    agg = collections.OrderedDict()
+    for d in per.values():
+        a = agg.setdefault(d['name'], collections.Counter())
+        a['n'] += 1
+        for k, v in d.items():
+            if k != 'name':
+                a[k] += v
+    print('| kernel | dispatches | executed MFMA TFLOP/s | MFMA busy |')
+    print('|---|---|---|---|')
+    for nm, a in agg.items():
+        mops = sum(a.get(k, 0.0) for k in a if k.startswith('SQ_INSTS_VALU_MFMA_MOPS'))
+        t = dur.get(nm, [])
+        tsum = sum(t) * a['n'] / max(len(t), 1) if t else 0.0
+        rate = FLOPS_PER_MOP * mops / tsum / 1e12 if tsum else float('nan')
+        busy = (a.get('SQ_VALU_MFMA_BUSY_CYCLES', 0.0) / (a['GRBM_GUI_ACTIVE'] * args.cu * 4)
+                if a.get('GRBM_GUI_ACTIVE') else float('nan'))
+        print('| `%s` | %d | %.2f | %.3f |' % (nm, a['n'], rate, busy))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest='cmd', required=True)
@@ -120,9 +183,20 @@ def main(argv=None):
     l_.add_argument('--write')
     l_.add_argument('--fetch-scale', type=float, default=1.0)
     l_.add_argument('--write-scale', type=float, default=1.0)
+    st = sub.add_parser('stats')
+    st.add_argument('csv')
+    st.add_argument('--top', type=int, default=16)
+    m = sub.add_parser('mfma')
+    m.add_argument('counters')
+    m.add_argument('trace')
+    m.add_argument('--cu', type=int, default=256)
     args = ap.parse_args(argv)
     if args.cmd == 'copy':
         _copy(args.gib)
+    elif args.cmd == 'stats':
+        _stats(args)
+    elif args.cmd == 'mfma':
+        _mfma(args)
     else:
         _launches(args)
 

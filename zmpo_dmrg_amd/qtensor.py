@@ -864,15 +864,11 @@ class Program:
         return sc
 
 
-_WORK = {}
-
-
 def _workspace(n, dtype):
-    buf = _WORK.get(dtype)
-    if buf is None or buf.numel() < n:
-        buf = dv.empty(max(n, 1), dtype)
-        _WORK[dtype] = buf
-    return buf
+    """Split-K workspace of n elements of dtype: a view of one byte buffer
+    shared by every launch and operand type (launches on one stream use it
+    one after the other)."""
+    return dv.typed(dv.scratch('splitk', int(max(n, 1)) * dv.ELSIZE[dtype]), 0, n, dtype)
 
 
 KERNEL = os.environ.get('ZMPO_KERNEL', 'shaped')
