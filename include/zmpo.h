@@ -368,6 +368,10 @@ int zmpo_memcpy(void* dst, const void* src, int64_t bytes, int kind, void* strea
 #define ZMPO_SEQ_STREAM 1
 #define ZMPO_SEQ_CHAIN 2
 #define ZMPO_SEQ_CHAIN16 3
+/* FP64 gather C[i] = A[idx[i]] (idx < 0 gives 0): idx in `tasks`, count in
+ * `nunits`, operands in slot_a and slot_c (the diagonal's environment
+ * gathers). */
+#define ZMPO_SEQ_GATHER 4
 typedef struct zmpo_seq_op {
   int32_t kind, shape, kt, upw, rb, atile, kmax, nmax;
   const void* tasks;
@@ -385,6 +389,34 @@ int zmpo_seq_destroy(zmpo_seq* s);
 int zmpo_seq_add(zmpo_seq* s, const zmpo_seq_op* op);
 int64_t zmpo_seq_length(const zmpo_seq* s);
 int zmpo_seq_run(const zmpo_seq* s, int dtype, void* const* slots, int nslots, void* stream);
+
+/* The §8(b) entry points of the path, each one recorded sequence run with
+ * its operands in the reference's order (no allocation, no host
+ * synchronisation: graph-capturable after one eager call).
+ *   zmpo_sigma_two_site  HVecQt (source/mpo_dmrg_kernelQt.py:153-319): the
+ *       plan of one (site, psi template, environment templates); L, R the
+ *       environments of all operators, W*_padded the zero-padded MPO sites,
+ *       psi in, sigma out (FP32 for dtype ZMPO_BF16), top1 / top3 / work
+ *       the intermediates and split-K workspace the plan was sized for.
+ *   zmpo_renorm  renormQt_H00 (kernelQt.py:633-767): site tensor values,
+ *       the environment in, the MPO site, intermediates t1 / t2, the new
+ *       environment out.
+ *   zmpo_hdiag  HDiagQt (kernelQt.py:30-127), FP64: the environments, the
+ *       MPO diagonal factors, scratch ldiag / rdiag / x, the diagonal out.
+ *   zmpo_davidson_residual  one Davidson correction vector (pdvdson.py:
+ *       240-283): r = W c - theta V c over nb basis vectors (rows of V and
+ *       W, leading dimension ld, device coefficients c), then, when diag is
+ *       given, r /= |diag - theta| + shift. */
+int zmpo_sigma_two_site(const zmpo_seq* plan, int dtype, const void* L, const void* R,
+                        const void* Wi_padded, const void* Wj_padded, const void* psi,
+                        void* sigma, void* top1, void* top3, void* work, void* stream);
+int zmpo_renorm(const zmpo_seq* plan, int dtype, const void* site, const void* env_in,
+                const void* W, void* t1, void* t2, void* env_out, void* work, void* stream);
+int zmpo_hdiag(const zmpo_seq* plan, const double* L, const double* R, const double* mpo_factors,
+               double* ldiag, double* rdiag, double* x, double* diag, void* work, void* stream);
+int zmpo_davidson_residual(int64_t n, const double* W, const double* V, int64_t ld, int nb,
+                           const double* coef, double theta, const double* diag, double shift,
+                           double* r, void* stream);
 
 #ifdef __cplusplus
 }

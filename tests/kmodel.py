@@ -267,6 +267,11 @@ class _CpuVecOps:
     def project_out(self, r, X, nx):
         return self.lincomb(r, X, self.dots(X, nx, r), -1.0, 1.0)
 
+    def residual(self, r, W, V, nb, coef, theta):
+        self.lincomb(r, W, coef[:nb], 1.0, 0.0)
+        self.lincomb(r, V, coef[:nb], -theta, 1.0)
+        return r
+
     def norm(self, x):
         return float(np.linalg.norm(_np(x)))
 

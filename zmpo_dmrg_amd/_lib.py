@@ -50,7 +50,7 @@ SEQ_OP_DTYPE = np.dtype([('kind', '<i4'), ('shape', '<i4'), ('kt', '<i4'), ('upw
                          ('slot_a', '<i4'), ('slot_b', '<i4'), ('slot_b2', '<i4'),
                          ('slot_c', '<i4'), ('slot_work', '<i4'), ('beta', '<i4'),
                          ('out_f32', '<i4'), ('pad', '<i4')])
-SEQ_SHAPED, SEQ_STREAM, SEQ_CHAIN, SEQ_CHAIN16 = 0, 1, 2, 3
+SEQ_SHAPED, SEQ_STREAM, SEQ_CHAIN, SEQ_CHAIN16, SEQ_GATHER = 0, 1, 2, 3, 4
 H2D, D2H, D2D = 0, 1, 2
 TILE = 64
 
@@ -59,7 +59,8 @@ EXPORTS = ['zmpo_last_error', 'zmpo_abi_version', 'zmpo_struct_sizes', 'zmpo_sym
            'zmpo_lincomb', 'zmpo_precond', 'zmpo_gather', 'zmpo_scatter', 'zmpo_convert',
            'zmpo_scratch_bytes', 'zmpo_streamgemm_static', 'zmpo_alloc', 'zmpo_free',
            'zmpo_memcpy', 'zmpo_syevd_sectors', 'zmpo_schedule_build', 'zmpo_seq_create', 'zmpo_seq_destroy', 'zmpo_seq_add',
-           'zmpo_seq_length', 'zmpo_seq_run']
+           'zmpo_seq_length', 'zmpo_seq_run', 'zmpo_sigma_two_site', 'zmpo_renorm',
+           'zmpo_hdiag', 'zmpo_davidson_residual']
 
 
 class ZmpoError(RuntimeError):
@@ -105,6 +106,11 @@ def lib():
     L.zmpo_free.argtypes = [vp]
     L.zmpo_memcpy.argtypes = [vp, vp, i64, i32, vp]
     L.zmpo_syevd_sectors.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.zmpo_sigma_two_site.argtypes = [vp, i32] + [vp] * 10
+    L.zmpo_renorm.argtypes = [vp, i32] + [vp] * 8
+    L.zmpo_hdiag.argtypes = [vp] * 10
+    L.zmpo_davidson_residual.argtypes = [i64, vp, vp, i64, i32, vp, ctypes.c_double, vp,
+                                         ctypes.c_double, vp, vp]
     L.zmpo_schedule_build.argtypes = [i64, vp, vp, i32, i32, i64, vp, vp, i64, vp, i64, vp]
     L.zmpo_seq_create.argtypes = [ctypes.POINTER(vp)]
     L.zmpo_seq_destroy.argtypes = [vp]

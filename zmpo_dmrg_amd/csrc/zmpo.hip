@@ -2736,8 +2736,19 @@ int zmpo_seq_destroy(zmpo_seq* s) {
 
 int zmpo_seq_add(zmpo_seq* s, const zmpo_seq_op* op) {
   if (!s || !op) return set_error(ZMPO_ERR_ARG, "seq_add: null pointer");
-  if (op->kind < ZMPO_SEQ_SHAPED || op->kind > ZMPO_SEQ_CHAIN16)
+  if (op->kind < ZMPO_SEQ_SHAPED || op->kind > ZMPO_SEQ_GATHER)
     return set_error(ZMPO_ERR_ARG, "seq_add: unknown launch kind");
+  if (op->kind == ZMPO_SEQ_GATHER) {
+    // FP64 gather C[i] = A[idx[i]], idx = op.tasks, n = op.nunits
+    if (op->slot_a < 0 || op->slot_c < 0 || (op->nunits > 0 && !op->tasks))
+      return set_error(ZMPO_ERR_ARG, "seq_add: gather needs slots a, c and an index array");
+    zmpo_seq_op g = *op;
+    g.slot_b = g.slot_a;  // no second operand
+    g.slot_b2 = -1;
+    g.slot_work = -1;
+    s->ops.push_back(g);
+    return ZMPO_OK;
+  }
   if (op->slot_a < 0 || op->slot_b < 0 || op->slot_c < 0 ||
       ((op->kind == ZMPO_SEQ_CHAIN || op->kind == ZMPO_SEQ_CHAIN16) && op->slot_b2 < 0))
     return set_error(ZMPO_ERR_ARG, "seq_add: missing operand slot");
@@ -2771,6 +2782,9 @@ int zmpo_seq_run(const zmpo_seq* s, int dtype, void* const* slots, int nslots, v
                              (const zmpo_task*)op.tasks, (const zmpo_rgroup*)op.groups, op.ngroups,
                              (const zmpo_pair*)op.pairs, A, B, C, W, op.beta, stream);
         break;
+      case ZMPO_SEQ_GATHER:
+        rc = zmpo_gather(ZMPO_F64, C, ZMPO_F64, A, (const int64_t*)op.tasks, op.nunits, stream);
+        break;
       default:
         rc = (op.kind == ZMPO_SEQ_CHAIN ? zmpo_chaingemm : zmpo_chaingemm16)(
             dt, op.rb, op.atile, op.kmax, op.nmax, (const zmpo_cunit*)op.units, op.nunits,
@@ -2781,6 +2795,47 @@ int zmpo_seq_run(const zmpo_seq* s, int dtype, void* const* slots, int nslots, v
     if (rc) return rc;
   }
   return ZMPO_OK;
+}
+
+// Named entry points over recorded sequences (the operand order of the
+// reference's calls).
+int zmpo_sigma_two_site(const zmpo_seq* plan, int dtype, const void* L, const void* R,
+                        const void* Wi_padded, const void* Wj_padded, const void* psi,
+                        void* sigma, void* top1, void* top3, void* work, void* stream) {
+  if (!plan || !L || !R || !Wi_padded || !Wj_padded || !psi || !sigma || !top1 || !top3)
+    return set_error(ZMPO_ERR_ARG, "sigma_two_site: null pointer");
+  void* slots[9] = {(void*)L,   (void*)R, (void*)Wi_padded, (void*)Wj_padded, (void*)psi,
+                    sigma,      top1,     top3,             work};
+  return zmpo_seq_run(plan, dtype, slots, 9, stream);
+}
+
+int zmpo_renorm(const zmpo_seq* plan, int dtype, const void* site, const void* env_in,
+                const void* W, void* t1, void* t2, void* env_out, void* work, void* stream) {
+  if (!plan || !site || !env_in || !W || !t1 || !t2 || !env_out)
+    return set_error(ZMPO_ERR_ARG, "renorm: null pointer");
+  void* slots[7] = {(void*)site, (void*)env_in, (void*)W, t1, t2, env_out, work};
+  return zmpo_seq_run(plan, dtype, slots, 7, stream);
+}
+
+int zmpo_hdiag(const zmpo_seq* plan, const double* L, const double* R, const double* mpo_factors,
+               double* ldiag, double* rdiag, double* x, double* diag, void* work, void* stream) {
+  if (!plan || !L || !R || !mpo_factors || !ldiag || !rdiag || !x || !diag)
+    return set_error(ZMPO_ERR_ARG, "hdiag: null pointer");
+  void* slots[8] = {(void*)L, (void*)R, (void*)mpo_factors, ldiag, rdiag, x, diag, work};
+  return zmpo_seq_run(plan, ZMPO_F64, slots, 8, stream);
+}
+
+int zmpo_davidson_residual(int64_t n, const double* W, const double* V, int64_t ld, int nb,
+                           const double* coef, double theta, const double* diag, double shift,
+                           double* r, void* stream) {
+  if (n < 0 || nb < 0) return set_error(ZMPO_ERR_ARG, "davidson_residual: bad sizes");
+  if (!r || (nb > 0 && (!W || !V || !coef)))
+    return set_error(ZMPO_ERR_ARG, "davidson_residual: null pointer");
+  int rc = zmpo_lincomb(ZMPO_F64, r, W, ld, nb, coef, 1.0, 0.0, n, stream);
+  if (rc) return rc;
+  rc = zmpo_lincomb(ZMPO_F64, r, V, ld, nb, coef, -theta, 1.0, n, stream);
+  if (rc || !diag) return rc;
+  return zmpo_precond(ZMPO_F64, r, diag, theta, shift, n, stream);
 }
 
 }  // extern "C"

@@ -425,10 +425,12 @@ class SigmaProgram:
                 work = _workspace(nwork, out_dtype(dtype)) if nwork else None
                 # the sigma's split-K slots and L x psi's share one workspace;
                 # they are FP32 in BF16 runs (L x psi never splits at these K)
-                slots = [dv.ptr(L), dv.ptr(R), dv.ptr(mpo.padded_buffer(self.isite, dtype)),
-                         dv.ptr(mpo.padded_buffer(self.isite + 1, dtype)), dv.ptr(psi),
-                         dv.ptr(sigma), dv.ptr(t1), dv.ptr(t3), dv.ptr(work)]
-                seq.run(dtype, slots, dv.stream())
+                _lib.check(_lib.lib().zmpo_sigma_two_site(
+                    seq._h, dtype, dv.ptr(L), dv.ptr(R),
+                    dv.ptr(mpo.padded_buffer(self.isite, dtype)),
+                    dv.ptr(mpo.padded_buffer(self.isite + 1, dtype)), dv.ptr(psi),
+                    dv.ptr(sigma), dv.ptr(t1), dv.ptr(t3), dv.ptr(work) if work is not None
+                    else None, dv.stream()))
                 return sigma
         first = True
         for st, chain in zip(self.stages, self.chains):
@@ -679,13 +681,41 @@ class DiagProgram:
         self.h2 = Program(t2, p2)
         self.flops = self.h1.flops + self.h2.flops
 
+    def sequence(self):
+        """The two environment gathers and two launches as one recorded
+        sequence over the slots (L, R, MPO factors, ldiag, rdiag, x, diag,
+        workspace), run by zmpo_hdiag; None when a launch cannot be
+        recorded."""
+        if '_seq' not in self.__dict__:
+            seq = _lib.Seq()
+            for idx, a, c in ((self.lidx, 0, 3), (self.ridx, 1, 4)):
+                rec = np.zeros(1, _lib.SEQ_OP_DTYPE)
+                rec['kind'] = _lib.SEQ_GATHER
+                rec['tasks'], rec['nunits'] = dv.ptr(idx), idx.numel()
+                rec['slot_a'], rec['slot_b'], rec['slot_b2'], rec['slot_c'] = a, a, -1, c
+                rec['slot_work'] = -1
+                seq.add(rec)
+            w1 = seq_add_program(seq, self.h1, 0, 3, 2, 5, 7)
+            w2 = seq_add_program(seq, self.h2, 0, 5, 4, 6, 7)
+            self._seq = (seq, max(w1, w2)) if (w1 is not None and w2 is not None) else None
+        return self._seq
+
     def apply(self, lenv, renv, out):
         if not self.nact:
             out[:] = 0
             return out
-        ld = gather(dv.empty(self.nl, 0), lenv.buffer(0), self.lidx)
-        rd = gather(dv.empty(self.nr, 0), renv.buffer(0), self.ridx)
-        x = dv.empty(self.nx, 0)
+        ld, rd, x = dv.empty(self.nl, 0), dv.empty(self.nr, 0), dv.empty(self.nx, 0)
+        sq = self.sequence() if SIGMA_SEQ and getattr(out, 'is_cuda', False) else None
+        if sq is not None:
+            seq, nwork = sq
+            work = _workspace(nwork, 0) if nwork else None
+            _lib.check(_lib.lib().zmpo_hdiag(
+                seq._h, dv.ptr(lenv.buffer(0)), dv.ptr(renv.buffer(0)), dv.ptr(self.pbuf),
+                dv.ptr(ld), dv.ptr(rd), dv.ptr(x), dv.ptr(out),
+                dv.ptr(work) if work is not None else None, dv.stream()))
+            return out
+        gather(ld, lenv.buffer(0), self.lidx)
+        gather(rd, renv.buffer(0), self.ridx)
         run_program(self.h1, ld, self.pbuf, x, 0)
         run_program(self.h2, x, rd, out, 0)
         return out
@@ -791,8 +821,10 @@ class RenormProgram:
         if sq is not None:
             seq, nwork = sq
             work = _workspace(nwork, dtype) if nwork else None
-            seq.run(dtype, [dv.ptr(x) for x in (site_values, E, W, t1, t2, out, work)],
-                    dv.stream())
+            _lib.check(_lib.lib().zmpo_renorm(
+                seq._h, dtype, dv.ptr(site_values), dv.ptr(E), dv.ptr(W), dv.ptr(t1),
+                dv.ptr(t2), dv.ptr(out), dv.ptr(work) if work is not None else None,
+                dv.stream()))
         else:
             for st in self.stages:
                 run_program(st[0], site_values, E, t1, dtype)

@@ -63,6 +63,14 @@ class _VecOps:
                                            float(alpha), float(beta), self.n, dv.stream()))
         return out
 
+    def residual(self, r, W, V, nb, coef, theta):
+        """r = W c - theta V c over the first nb rows (zmpo_davidson_residual)."""
+        c = dv.upload(np.asarray(coef[:nb], dtype=np.float64), 0)
+        _lib.check(_lib.lib().zmpo_davidson_residual(self.n, dv.ptr(W), dv.ptr(V), W.stride(0),
+                                                     nb, dv.ptr(c), float(theta), None, 0.0,
+                                                     dv.ptr(r), dv.stream()))
+        return r
+
     def norm(self, x):
         return float(np.sqrt(max(self.dots(x.view(1, -1), 1, x)[0], 0.0)))
 
@@ -195,8 +203,7 @@ class EigenSolver:
             eigs = teig.copy()
             rconv, rindx = [], []
             for i in range(neig):
-                ops.lincomb(R[i], W, vr[i], 1.0, 0.0)
-                ops.lincomb(R[i], V, vr[i], -eigs[i], 1.0)
+                ops.residual(R[i], W, V, nb, vr[i], eigs[i])
                 rn = ops.norm(R[i])
                 rconv.append(rn <= max(self.crit_vec, self.tol_scale * max(1.0, abs(eigs[i]))))
                 if not rconv[i] and not econv[i]:
