@@ -172,6 +172,11 @@ int zmpo_wavegemm(int dtype, int rb, const zmpo_task* tasks, const zmpo_unit* un
 #define ZMPO_PANEL_128x128 9
 #define ZMPO_PANEL_32x64 10
 #define ZMPO_PANEL_64x32 11
+/* 32 x 160: one row band of a ~130-160 wide output block per workgroup, so
+ * the A rows are read once (sigma x R).  Only for programs whose every pair
+ * has a_sk == 1 and b_sk == 1 (k-contiguous operands); other programs give
+ * wrong results with it. */
+#define ZMPO_PANEL_32x160 12
 
 /* LDS-staged grouped GEMM with block-shaped tiles: one workgroup per
  * zmpo_unit (unit m0/n0 step by the tile shape), same split-K groups and

@@ -77,3 +77,14 @@ def test_shipped_table_is_well_formed():
         role, dt = e['signature'][:2]
         assert role in at.CANDIDATES and dt in (0, 1, 2, 3)
         assert (e['kernel'], e['shape']) in at.CANDIDATES[role]
+
+
+def test_kmajor_tile_needs_k_contiguous_operands():
+    kmaj = _program([64], [64], [[64, 32]])            # a_sk = 1, b_sk = N (row-major B)
+    kmaj.tiles[0] = ('shaped', 12)
+    assert kmaj.launch_choice(0) == ('shaped', 2)       # falls back: B is not k-contiguous
+    t, p = kmaj.tasks_host.copy(), kmaj.pairs_host.copy()
+    p['b_sk'], p['b_snlo'] = 1, p['K']
+    ok = pq.Program(t, p)
+    ok.tiles[0] = ('shaped', 12)
+    assert ok.kmajor_ok() and ok.launch_choice(0) == ('shaped', 12)

@@ -167,3 +167,30 @@ def test_panel_sigma_matches_default_tiles(gpu, dtype):
         sp.apply(c.mpo, L_[isite - 1], R_[isite + 2], dv.upload(v, dtype), s1, dtype)
         err = np.abs(dv.download(s1).astype(np.float64) - want).max()
         assert err < tol, (shape, dtype, err)
+
+
+@pytest.mark.parametrize('dtype', [L.F64, L.F32, L.ZMPO_BF16_F32])
+def test_panel_32x160_on_k_contiguous_operands(gpu, dtype):
+    """The 32 x 160 tile (k-contiguous operands only) on single-block and
+    split-K programs against numpy; other programs fall back to 32 x 32."""
+    store = {L.F64: 0, L.F32: 1, L.ZMPO_BF16_F32: 2}[dtype]
+    cstore = {L.F64: 0, L.F32: 1, L.ZMPO_BF16_F32: 1}[dtype]
+    for M, N, K in [(134, 134, 2500), (37, 152, 300), (1, 1, 1), (200, 170, 33)]:
+        rng = np.random.default_rng(M + N + K)
+        A = rng.standard_normal(M * K)
+        B = rng.standard_normal(K * N)
+        if store == 2:
+            A, B = bf16(A), bf16(B)
+        prog = _dense_program(M, N, K, False, True)     # A (M, K), B stored (N, K)
+        assert prog.kmajor_ok()
+        prog.tiles[dtype] = ('shaped', 12)
+        prog.kchunk = 700                                 # split K over several units
+        assert prog.launch_choice(dtype) == ('shaped', 12)
+        C = dv.zeros(M * N, cstore)
+        pq.run_program(prog, dv.upload(A, store), dv.upload(B, store), C, dtype)
+        Am, Bm = A.reshape(M, K), B.reshape(N, K).T
+        ref = Am @ Bm
+        mag = np.abs(Am) @ np.abs(Bm)
+        got = dv.download(C).astype(np.float64).reshape(M, N)
+        tol = 1e-15 * K + 1e-15 if dtype == L.F64 else 1e-6 * K + 1e-6
+        assert np.all(np.abs(got - ref) <= tol * mag + 1e-30), (dtype, M, N, K)

@@ -50,7 +50,7 @@ CANDIDATES = {
                 ('shaped', 7), ('shaped', 10), ('shaped', 11)],
     # top3 x R: square sigma blocks, hundreds of long pairs each (split-K)
     'sigma_r': [('stream', 2), ('shaped', 2), ('shaped', 3), ('shaped', 6), ('shaped', 8),
-                ('shaped', 10), ('shaped', 11)],
+                ('shaped', 10), ('shaped', 11), ('shaped', 12)],
 }
 # the three launches of a renormalisation step (site x environment, MPO x
 # that, x site into the new environment)
@@ -129,9 +129,11 @@ def tune(prog, role, dtype, launch, reps=REPS):
     score = {}
     for kernel, shape in CANDIDATES[role]:
         prog.tiles[dtype] = (kernel, shape)
+        k_eff, s_eff = prog.launch_choice(dtype)
+        if s_eff != shape:
+            continue                  # the program cannot use this tile
         launch()                      # builds and uploads the schedule
         dv.sync()
-        k_eff = prog.launch_choice(dtype)[0]
         nunits = (prog._sched or {}).get((k_eff, shape), {}).get('nunits', 0)
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
@@ -144,13 +146,13 @@ def tune(prog, role, dtype, launch, reps=REPS):
         score[(kernel, shape)] = times[(kernel, shape)] + HOST_MS_PER_UNIT * nunits
         prog.__dict__.setdefault('_tune_units', {})[(kernel, shape)] = nunits
     best = min(score, key=score.get)
+    prog.tiles[dtype] = best
     # keep only the chosen schedule on the device
     if prog._sched:
-        keep = prog.launch_choice(dtype, best[0])[0], best[1]
+        keep = prog.launch_choice(dtype)
         for key in list(prog._sched):
             if key != keep:
                 del prog._sched[key]
-    prog.tiles[dtype] = best
     return times, best
 
 

@@ -771,6 +771,9 @@ __device__ __forceinline__ unsigned stg_scale(unsigned v, int mode, float sc) {
 // rf + i * rs at the fixed k = kb (16 / 32 consecutive threads cover a
 // row's k-tile), otherwise row rf at k = kb + i * ks (consecutive threads
 // cover consecutive rows).  off[i] is the element's offset at k-tile 0.
+// The second form covers the tile only when ROWS divides 256: tiles with
+// other row counts (ZMPO_PANEL_32x160) take k-contiguous operands only,
+// which the host checks before choosing them.
 template <typename T, int ROWS>
 struct PanelLoader {
   static constexpr int KT = Panel<T>::KT;
@@ -1003,6 +1006,7 @@ int dispatch_shaped(int shape, const zmpo_task* tasks, const zmpo_unit* units, i
     case ZMPO_PANEL_128x128: return launch_panel<T, TC, 128, 128>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
     case ZMPO_PANEL_32x64: return launch_panel<T, TC, 32, 64>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
     case ZMPO_PANEL_64x32: return launch_panel<T, TC, 64, 32>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
+    case ZMPO_PANEL_32x160: return launch_panel<T, TC, 32, 160>(tasks, units, nunits, groups, ngroups, pairs, A, B, C, work, beta, s);
     default: return set_error(ZMPO_ERR_ARG, "shapedgemm: unknown tile shape");
   }
 }

@@ -583,8 +583,11 @@ def make_schedule_np(tasks, pairs, rb, kchunk=2048, reuse_key=None, tile=None):
 SHAPES = {0: (64, 16), 1: (128, 16), 2: (32, 32), 3: (64, 32), 4: (16, 16), 5: (64, 64),
           # register-blocked panel tiles (zmpo_shapedgemm ZMPO_PANEL_*): same
           # schedule, 2x2 waves, k-contiguous LDS, BF16 on the 16x16x32 MFMA
-          6: (64, 64), 7: (128, 64), 8: (64, 128), 9: (128, 128), 10: (32, 64), 11: (64, 32)}
-PANEL_SHAPES = (6, 7, 8, 9, 10, 11)
+          6: (64, 64), 7: (128, 64), 8: (64, 128), 9: (128, 128), 10: (32, 64), 11: (64, 32),
+          12: (32, 160)}
+PANEL_SHAPES = (6, 7, 8, 9, 10, 11, 12)
+# panel tiles whose row count does not divide 256: k-contiguous operands only
+KMAJOR_SHAPES = (12,)
 # summed K per unit of the panel tiles: each unit of a long reduction (sigma
 # x R: hundreds of K ~ 2500 pairs per output block) runs ~10 pairs, which
 # keeps the split-K workspace to a few GB at K = 36, D = 1000
@@ -699,7 +702,7 @@ class Program:
         self._host = None
         self._dev = None
         self._sched = None
-        self._rb = self._shape = self._small = None
+        self._rb = self._shape = self._small = self._kmajor = None
         rows = np.repeat(np.arange(self.ntasks), (tasks['p_end'] - tasks['p_begin']).astype(np.int64))
         self.flops = int(np.sum(2 * tasks['M'][rows].astype(np.int64) * tasks['N'][rows]
                                 * pairs['K'].astype(np.int64))) if self.npairs else 0
@@ -759,6 +762,15 @@ class Program:
                 _max_row_offset(p['a_mlo'], p['a_smhi'], p['a_smlo'], t['M'][rows]),
                 _max_row_offset(p['b_nlo'], p['b_snhi'], p['b_snlo'], t['N'][rows])) < 2 ** 31 - 1)
         return self._small
+
+    def kmajor_ok(self):
+        """Whether every pair reads both operands k-contiguously (a_sk ==
+        b_sk == 1), as the KMAJOR_SHAPES tiles need."""
+        if self._kmajor is None:
+            p = self.pairs_host
+            self._kmajor = bool(self.npairs == 0 or (np.all(p['a_sk'] == 1)
+                                                     and np.all(p['b_sk'] == 1)))
+        return self._kmajor
 
     def skinny_ok(self):
         """Whether zmpo_skinnygemm applies: K-contiguous single-level A rows."""
@@ -822,6 +834,8 @@ class Program:
         shape = self.shape
         if kernel is None:
             kernel, shape = self.tiles.get(dtype, (self.prefer or KERNEL, self.shape))
+        if shape in KMAJOR_SHAPES and not self.kmajor_ok():
+            shape = 2
         if kernel == 'stream' and (not self.small_blocks or shape in PANEL_SHAPES):
             kernel = 'shaped'
         if kernel == 'skinny' and not self.skinny_ok():
