@@ -70,3 +70,47 @@ def test_normal2_run_reaches_fci_on_small_case():
     h, eri = gen_spatial(4, seed=3)
     efci = fci_energy(h, eri, 4)[0]
     assert abs(es[-1] - efci) < 1e-8
+
+
+def test_spilled_environments_change_no_energy():
+    """An environment budget below one sweep's environments moves the ones
+    read farthest ahead to host memory and back: the energies stay bit for
+    bit, the steps' two environments are resident when read, and the
+    resident bytes stay within the budget plus those two."""
+    from parity_util import Case
+    with host_model():
+        c = Case(K=5, D=8, seed=5, sweeps=0)
+        ref = pd.MPODMRG(c.i1, c.i2, c.nelec, Dmax=8, tol=1e-9, mpo=c.mpo)
+        e_ref = pd.run(ref, 1)
+        big = max(e.nbytes() for e in ref.envL.values())
+        budget = big
+        d = pd.MPODMRG(c.i1, c.i2, c.nelec, Dmax=8, tol=1e-9, mpo=c.mpo, env_budget=budget)
+        seen = []
+        orig = pd.fit_envs
+
+        def spy(dmrg, isite, status, envL=None):
+            orig(dmrg, isite, status, envL)
+            if envL is None:
+                dev, host = pd.env_bytes(dmrg)
+                need = [dmrg.envL[isite - 1], dmrg.envR[isite + 2]]
+                pending = sum(e.nbytes() for e in need if not e.resident())
+                seen.append((dev + pending, host))
+        pd.fit_envs = spy
+        try:
+            e = pd.run(d, 1)
+        finally:
+            pd.fit_envs = orig
+    assert e == e_ref
+    assert max(s[0] for s in seen) <= budget + 2 * big
+    assert max(s[1] for s in seen) > 0                  # something was spilled
+    assert d.timing.get('spill', 0.0) > 0.0
+
+
+def test_host_budget_overflow_is_an_error():
+    import pytest
+    from parity_util import Case
+    with host_model():
+        c = Case(K=5, D=8, seed=5, sweeps=0)
+        with pytest.raises(MemoryError):
+            pd.MPODMRG(c.i1, c.i2, c.nelec, Dmax=8, tol=1e-9, mpo=c.mpo, env_budget=0,
+                       host_budget=1)

@@ -61,7 +61,7 @@ class MPODMRG:
 
     def __init__(self, int1e, int2e, nelec, sz=0.0, Dmax=100, tol=1e-8, crit_vec=1e-8,
                  occun=None, dtype=0, mpo=None, budget=kernelQt.DEFAULT_BUDGET,
-                 keep_envs=False):
+                 keep_envs=False, env_budget=None, host_budget=None):
         self.fhop = mpo if mpo is not None else MPOStore(int1e, int2e)
         self.nsite = self.fhop.nsite
         self.sbas = self.fhop.sbas
@@ -88,6 +88,10 @@ class MPODMRG:
         # not read again (the next half-sweep rebuilds them before use), so
         # about one environment per bond is resident instead of two
         self.keep_envs = keep_envs
+        # env_budget (bytes): device memory the environments may hold; the
+        # ones read farthest ahead move to host memory (at most host_budget
+        # bytes) until the rest fit.  None keeps every environment resident.
+        self.env_budget, self.host_budget = env_budget, host_budget
         K = self.nsite
         occun = default_occupation(K, nelec, sz) if occun is None else occun
         self.qnuml, self.qnumr, s0 = mps0occun(K, occun)
@@ -98,7 +102,69 @@ class MPODMRG:
             self.lmps[i] = q
         # initOps: right boundary + every left environment (util.py:193-214)
         self.envR = {K: kernelQt.boundary_store(self.nops)}
-        self.envL, _ = kernelQt.genHops(self.fhop, self.lmps, 'L')
+        self.envL, _ = kernelQt.genHops(self.fhop, self.lmps, 'L', fit=self._fit_built)
+
+    def _fit_built(self, envs, i):
+        # left environments built for the 'R' half-sweep that follows
+        fit_envs(self, i + 1, 'R', envL=envs)
+
+
+def env_bytes(dmrg):
+    """(device bytes, host bytes) of the FP64 environment values."""
+    dev = host = 0
+    for env in (dmrg.envL, dmrg.envR):
+        for e in env.values():
+            if e.resident():
+                dev += e.nbytes()
+            else:
+                host += e.nbytes()
+    return dev, host
+
+
+def fit_envs(dmrg, isite, status, envL=None):
+    """Hold the resident environments within dmrg.env_budget bytes before
+    the two-site step at isite (half-sweep ``status``), counting the two it
+    reads (envL[isite-1], envR[isite+2]) as resident.  The environments
+    whose next read lies farthest ahead move to host memory first: in an
+    'L' half-sweep the left ones (read in the next 'R' half-sweep, smallest
+    bond last) and then the right ones from the far end; an 'R' half-sweep
+    mirrors this.  The files of mpo_dmrg_opt.py:196-238 play this part in
+    the reference, where every environment lives on disk."""
+    budget = getattr(dmrg, 'env_budget', None)
+    if budget is None:
+        return
+    envL = dmrg.envL if envL is None else envL
+    envR = dmrg.envR
+    need = {('L', isite - 1), ('R', isite + 2)}
+    left = [('L', j) for j in sorted(envL)]
+    right = [('R', j) for j in sorted(envR, reverse=True)]
+    order = left + right if status == 'L' else right + left
+    store = {'L': envL, 'R': envR}
+    dev = host = 0
+    for side, j in left + right:
+        e = store[side][j]
+        if e.resident() or (side, j) in need:
+            dev += e.nbytes()
+        else:
+            host += e.nbytes()
+    hbudget = getattr(dmrg, 'host_budget', None)
+    for side, j in order:
+        if dev <= budget:
+            break
+        e = store[side][j]
+        if (side, j) in need or not e.resident():
+            continue
+        n = e.nbytes()
+        if hbudget is not None and host + n > hbudget:
+            raise MemoryError('environments need %.1f GiB of device memory and %.1f GiB of host '
+                              'memory; budgets %.1f / %.1f GiB' % (
+                                  dev / 2 ** 30, (host + n) / 2 ** 30, budget / 2 ** 30,
+                                  hbudget / 2 ** 30))
+        t0 = time.time()
+        e.spill()
+        dmrg.timing['spill'] = dmrg.timing.get('spill', 0.0) + time.time() - t0
+        dev -= n
+        host += n
 
 
 def restart_from_left_mps(dmrg, sites, qnuml):
@@ -111,7 +177,8 @@ def restart_from_left_mps(dmrg, sites, qnuml):
         dmrg.lmps[i] = sites[i]
     for i in range(K + 1):
         dmrg.qnuml[i] = np.asarray(qnuml[i])
-    dmrg.envL, _ = kernelQt.genHops(dmrg.fhop, dmrg.lmps, 'L')
+    dmrg.envL, dmrg.envR = {}, {}      # released before the new ones are built
+    dmrg.envL, _ = kernelQt.genHops(dmrg.fhop, dmrg.lmps, 'L', fit=dmrg._fit_built)
     dmrg.envR = {K: kernelQt.boundary_store(dmrg.nops)}
     # the first step of the next 'R' half-sweep (sites K-2, K-1) starts from
     # the MPS's own two-site wavefunction
@@ -198,6 +265,7 @@ def sweep(dmrg, status, max_steps=None):
             break
         if release:
             release_dead_envs(dmrg, isite, status, after=False)
+        fit_envs(dmrg, isite, status)
         flst = [[dmrg.envL[isite - 1], dmrg.envR[isite + 2]]]
         t0 = time.time()
         nm, eigs, sigs, dwts, qred, site, srotR = ci_solver(dmrg, isite, 2, actlst, flst, status)

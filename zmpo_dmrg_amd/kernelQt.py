@@ -23,6 +23,7 @@ Device layouts (leg storage order inside each block):
 import collections
 import hashlib
 import os
+import time
 
 import numpy as np
 
@@ -89,6 +90,10 @@ def op_case(mpo, p, isite):
 
 
 # ------------------------------------------------------------------ stores
+# environment moves between device and host memory (EnvStore.spill / buffer)
+SPILL = {'download_s': 0.0, 'download_bytes': 0, 'upload_s': 0.0, 'upload_bytes': 0}
+
+
 class EnvStore:
     """Environments of one bond for every op: ``meta[p]`` (QTensor, ENV_PERM)
     and one device buffer ``buf`` with op p at ``off[p]`` (replaces the HDF5
@@ -106,7 +111,10 @@ class EnvStore:
         """The values in one precision (0 FP64, 1 FP32, 2 BF16); lower
         precisions are converted once and kept until drop_low()."""
         if isinstance(self.buf, np.ndarray):
+            t0 = time.time()
             self.buf = dv.upload(self.buf, 0)
+            SPILL['upload_s'] += time.time() - t0
+            SPILL['upload_bytes'] += self.nbytes()
         if dtype == 0:
             return self.buf
         if dtype not in self._lo:
@@ -118,6 +126,25 @@ class EnvStore:
 
     def drop_low(self):
         """Release the low-precision copies."""
+        self._lo = {}
+
+    def nbytes(self):
+        """Bytes of the FP64 values, wherever they are held."""
+        b = self.buf
+        return int(b.nbytes) if isinstance(b, np.ndarray) else int(b.numel() * b.element_size())
+
+    def resident(self):
+        """Whether the FP64 values are in device memory."""
+        return not isinstance(self.buf, np.ndarray)
+
+    def spill(self):
+        """Move the FP64 values to host memory and release every device
+        copy; the next buffer() call uploads them again."""
+        if self.resident():
+            t0 = time.time()
+            self.buf = dv.download(self.buf)
+            SPILL['download_s'] += time.time() - t0
+            SPILL['download_bytes'] += self.nbytes()
         self._lo = {}
 
     def op_tensor(self, p):
@@ -846,19 +873,25 @@ def renorm(mpo, site_idx, qsite, env_in, status, dtype=0, budget=DEFAULT_BUDGET)
     return prog.apply(mpo, v, env_in, dtype), prog
 
 
-def genHops(mpo, sites, status, nqnum=2):
-    """All environments from an MPS (initQt.py:88-271).  Returns (envs, exphop)."""
+def genHops(mpo, sites, status, nqnum=2, fit=None):
+    """All environments from an MPS (initQt.py:88-271).  Returns (envs, exphop).
+    ``fit(envs, i)``, when given, runs after environment i is built (the
+    sweep driver's device-memory policy; it keeps envs[i] resident)."""
     K = mpo.nsite
     envs = {}
     if status == 'L':
         envs[-1] = boundary_store(mpo.nops, nqnum)
         for i in range(K):
             envs[i], _ = renorm(mpo, i, sites[i], envs[i - 1], 'L')
+            if fit is not None:
+                fit(envs, i)
         last = envs[K - 1]
     else:
         envs[K] = boundary_store(mpo.nops, nqnum)
         for i in range(K - 1, -1, -1):
             envs[i], _ = renorm(mpo, i, sites[i], envs[i + 1], 'R')
+            if fit is not None:
+                fit(envs, i)
         last = envs[0]
     vals = last.buf if isinstance(last.buf, np.ndarray) else dv.download(last.buf)
     exphop = np.array([vals[last.off[p]] if last.meta[p].size_allowed else 0.0

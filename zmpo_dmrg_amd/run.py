@@ -28,7 +28,13 @@ def main(argv=None):
     ap.add_argument('--sweeps', type=int, default=20)
     ap.add_argument('--tol', type=float, default=1e-7)
     ap.add_argument('--noise', type=float, default=0.0, help='noise of the fixed schedule')
-    ap.add_argument('--budget-gb', type=float, default=64.0)
+    ap.add_argument('--budget-gb', type=float, default=64.0,
+                    help='device bytes of the sigma intermediates per op chunk')
+    ap.add_argument('--env-budget-gb', type=float, default=None,
+                    help='device bytes the environments may hold; the rest wait in host '
+                         'memory (default: all resident)')
+    ap.add_argument('--host-budget-gb', type=float, default=200.0,
+                    help='host bytes the spilled environments may hold')
     ap.add_argument('--save', default=None)
     ap.add_argument('--load', default=None)
     ap.add_argument('--start-m', type=int, default=None,
@@ -47,8 +53,10 @@ def main(argv=None):
     K = args.K
     h, eri = gen_spatial(K, seed=args.seed)
     t0 = time.time()
+    gb = lambda x: None if x is None else int(x * 2 ** 30)
     d = pd.MPODMRG(*spin_orbital(h, eri), K, 0.0, Dmax=args.D, tol=args.tol, dtype=args.dtype,
-                   budget=int(args.budget_gb * 2 ** 30))
+                   budget=gb(args.budget_gb), env_budget=gb(args.env_budget_gb),
+                   host_budget=gb(args.host_budget_gb))
     if args.load:
         sites, qnums = mpsio.load(args.load)
         pd.restart_from_left_mps(d, sites, qnums)
@@ -66,10 +74,12 @@ def main(argv=None):
 
     def step_log(status, isite, eig, dt, nsig):
         ls = getattr(d, 'last_step', {})
-        print('[run]   %s site %2d E=%.10f %.2fs sigma %d mem %.1f GiB ndim0 %s sigma %.1f GFLOP '
-              'sectors %s dims %s' % (status, isite, eig, dt, nsig, gib(), ls.get('ndim0'),
-                                      ls.get('sigma_gflop', 0.0), ls.get('sectors'),
-                                      ls.get('dims')), flush=True)
+        edev, ehost = pd.env_bytes(d)
+        print('[run]   %s site %2d E=%.10f %.2fs sigma %d mem %.1f GiB (environments %.1f, '
+              'host %.1f) ndim0 %s sigma %.1f GFLOP sectors %s dims %s' % (
+                  status, isite, eig, dt, nsig, gib(), edev / 2 ** 30, ehost / 2 ** 30,
+                  ls.get('ndim0'), ls.get('sigma_gflop', 0.0), ls.get('sectors'),
+                  ls.get('dims')), flush=True)
 
     d.step_log = step_log
     out = {'energies': [], 'sweep_s': [], 'Dmax': [], 'nsigma': []}
@@ -83,6 +93,11 @@ def main(argv=None):
             isweep, d.Dmax, d.crit_e, d.noise, eav, dt, d.nsigma, gib(True)), flush=True)
         print('[run]   timing so far: %s' % ' '.join('%s %.1f' % kv for kv in sorted(
             d.timing.items())), flush=True)
+        from .kernelQt import SPILL
+        if SPILL['download_bytes']:
+            print('[run]   environment moves: %.1f GiB to host in %.1fs, %.1f GiB back in %.1fs' % (
+                SPILL['download_bytes'] / 2 ** 30, SPILL['download_s'],
+                SPILL['upload_bytes'] / 2 ** 30, SPILL['upload_s']), flush=True)
         if args.save:
             mpsio.save(args.save, {i: d.lmps[i] for i in range(K)},
                        [np.asarray(d.qnuml[i]) for i in range(K + 1)])
@@ -122,10 +137,12 @@ def compare_precisions(d, args, step_log):
         n0, n640 = d.nsigma, getattr(d, 'nsigma64', 0)
         t0 = time.time()
         r1 = pd.sweep(d, 'R')
+        t1 = time.time()
         r2 = pd.sweep(d, 'L')
         dt_s = time.time() - t0
         steps = [float(e[0]) for e in r1[3]] + [float(e[0]) for e in r2[3]]
-        ent = {'sweep_s': dt_s, 'energy': min(r1[2], r2[2]), 'steps': steps,
+        ent = {'sweep_s': dt_s, 'half_s': [t1 - t0, dt_s - (t1 - t0)],
+               'energy': min(r1[2], r2[2]), 'steps': steps,
                'nsigma_low': d.nsigma - n0 if dt else 0,
                'nsigma_f64': (getattr(d, 'nsigma64', 0) - n640) if dt else d.nsigma - n0,
                'timing': dict(d.timing)}
@@ -135,11 +152,15 @@ def compare_precisions(d, args, step_log):
             ent['dE_first_step'] = abs(steps[0] - base['steps'][0])
             ent['dE_sweep'] = abs(ent['energy'] - base['energy'])
             ent['speedup'] = base['sweep_s'] / dt_s
+            ent['speedup_half'] = [a / b for a, b in zip(base['half_s'], ent['half_s'])]
         res[dt] = ent
-        print('[run] compare dtype %d: sweep %.1fs E=%.10f %s' % (
-            dt, dt_s, ent['energy'], ' '.join('%s=%.3g' % (k, v) for k, v in ent.items()
-                                               if k in ('dE_first_step', 'dE_sweep', 'speedup',
-                                                        'nsigma_low', 'nsigma_f64'))), flush=True)
+        print('[run] compare dtype %d: sweep %.1fs (R %.1fs, L %.1fs) E=%.10f %s' % (
+            dt, dt_s, ent['half_s'][0], ent['half_s'][1], ent['energy'],
+            ' '.join('%s=%.3g' % (k, v) for k, v in ent.items()
+                     if k in ('dE_first_step', 'dE_sweep', 'speedup', 'nsigma_low',
+                              'nsigma_f64'))), flush=True)
+        print('[run]   timing: %s' % ' '.join('%s %.1f' % kv for kv in sorted(d.timing.items())),
+              flush=True)
     os.remove(path)
     return res
 
