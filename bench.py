@@ -37,11 +37,11 @@ def build_workload(K, D, seed, dtype, budget, fused=True, extra=None, structure=
 
 
 def time_stages(prog, mpo, L, R, psi, sig, dtype, reps=3):
-    """dtype 2: BF16 operands, FP32 sigma (the *R launch runs as ZMPO_BF16_F32)."""
     """Per-launch average time of each sigma kernel, with HIP events recorded
     on the stream the kernels are launched on, plus its algorithmic FLOPs and
     bytes.  Launches (per op chunk): L*psi, then either the chained
-    W[i]*W[i+1] launch or its two separate stages, then *R into sigma."""
+    W[i]*W[i+1] launch or its two separate stages, then *R into sigma.
+    dtype 2: BF16 operands, FP32 sigma (the *R launch runs as ZMPO_BF16_F32)."""
     from zmpo_dmrg_amd import device as dv
     from zmpo_dmrg_amd.kernelQt import out_dtype
     from zmpo_dmrg_amd.qtensor import run_chain, run_program
@@ -325,9 +325,12 @@ def main():
         prog.apply(mpo, L, R, psi, sig, 0)
         s64 = dv.download(sig)
         err = float(np.linalg.norm(dv.download(sig32).astype(np.float64) - s64) / np.linalg.norm(s64))
+        _, st_ms32, _, _ = time_stages(prog, mpo, L, R, psi32, sig32, 1)
         prec = {'f32_ms_per_step': ms32, 'f32_gflops': prog.flops / ms32 / 1e6,
-                'f32_speedup_vs_f64': ms_step / ms32, 'f32_rel_err': err}
-        log('FP32 sigma: %.3f ms/step (%.2fx FP64), relative error %.2e' % (ms32, ms_step / ms32, err))
+                'f32_speedup_vs_f64': ms_step / ms32, 'f32_rel_err': err,
+                'f32_stage_ms': [round(x, 4) for x in st_ms32]}
+        log('FP32 sigma: %.3f ms/step (%.2fx FP64), relative error %.2e, stages %s' % (
+            ms32, ms_step / ms32, err, st_ms32))
         # BF16 operands, FP32 accumulation (BF16 MFMA), FP32 sigma
         psi16 = dv.upload(psi_h, 2)
         prog.apply(mpo, L, R, psi16, sig32, 2)

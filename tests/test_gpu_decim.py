@@ -93,12 +93,16 @@ def test_device_decimation_matches_host(gpu, status, neig):
         assert np.abs(ph - pd_ * sign[None, None, :]).max() < 1e-10
 
 
-def test_device_decimation_sweeps_match_host(gpu):
+@pytest.mark.parametrize('noise', [0.0, 1e-4])
+def test_device_decimation_sweeps_match_host(gpu, noise):
+    """With the noise term too (the pRDMQt operator term added to the
+    device density matrix)."""
     K, D = 6, 12
     out = {}
     for mode in ('host', 'device'):
         i1, i2 = gen_integrals(K, seed=5)
         d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=D, tol=1e-10)
+        d.noise = noise
         saved = pdot.DECIM
         pdot.DECIM = mode
         try:
@@ -111,3 +115,33 @@ def test_device_decimation_sweeps_match_host(gpu):
     assert np.abs(np.array(eh) - np.array(ed)).max() < 1e-9, (eh, ed)
     for a, b in zip(out['host'][1], out['device'][1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize('status', ['L', 'R'])
+def test_device_noise_term_matches_host(gpu, status):
+    """prdm_noise_device (C template gathered from the dense vectors, Gram
+    blocks scattered into the dense matrix on the device) against the host
+    restatement prdm_noise on the same two-site vectors."""
+    from zmpo_dmrg_amd import kernelQt as kq
+    K, D = 6, 12
+    i1, i2 = gen_integrals(K, seed=3)
+    d = pdmrg.MPODMRG(i1, i2, K, 0.0, Dmax=D, tol=1e-9, keep_envs=True)
+    d.noise = 1e-4
+    pdmrg.run(d, 1)
+    isite = 2
+    prjmap, n = _prepare(d, isite, status)
+    neig = 2
+    key = list(d.qsectors.keys())[0]
+    d.qsectors = {key: neig}          # both paths sum over the sector's states
+    rng = np.random.default_rng(11)
+    vec = rng.standard_normal((neig, n))
+    civecs = np.zeros((neig, d.dims[3]))
+    civecs[:, prjmap] = vec
+    ldim, _, rdim, _ = d.dims
+    flst = [[d.envL[isite - 1], d.envR[isite + 2]]]
+    info = [d, isite, 2, flst, status, True]
+    shp = (neig, ldim, 4, 4 * rdim) if status == 'L' else (neig, ldim * 4, 4, rdim)
+    want = 0.5 * kq.prdm_noise(civecs.reshape(shp), info)
+    got = dv.download(kq.prdm_noise_device(dv.upload(civecs.reshape(-1), 0), neig, info, 0.5))
+    assert got.size == want.size
+    assert np.abs(got.reshape(want.shape) - want).max() <= 1e-12 * max(1.0, np.abs(want).max())

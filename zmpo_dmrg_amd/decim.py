@@ -23,10 +23,12 @@ For one two-site step with the Davidson solutions on the device:
    site's dense tensor) by device launches; the guess stays on the device
    for the next step's Davidson start vector.
 
+With the noise term (dmrg.noise >= 1e-10) the caller adds noise times the
+operator term of pRDMQt (kernelQt.prdm_noise, dense over the kept side) to
+rho before step 2's sector gather, as dotutil.py:312-333 adds it to the RDM.
 The host copies of the reference's path (dense civecs, the RDM, eigh,
-srotR and the guess in numpy) are gone from this path; with the noise term
-(dmrg.noise >= 1e-10) or an all-zero density matrix dot.decimation runs
-instead.
+srotR and the guess in numpy) are gone from this path; with an all-zero
+density matrix dot.decimation runs instead.
 """
 import numpy as np
 
@@ -102,7 +104,9 @@ class DeviceDecimation:
         run_program(prog, self.C, self.C, rho, 0)
         return rho
 
-    def run(self, classes, thresh, Dcut):
+    def run(self, classes, thresh, Dcut, extra=None):
+        """Steps 2-5; ``extra`` (device float64 m x m, or None) is added to
+        rho.  False when rho is all zero (the host path handles it)."""
         from .dot import WZERO, canonical_null, truncation_order
         m = self.m
         keys, groups = [], {}
@@ -118,6 +122,10 @@ class DeviceDecimation:
         woff = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64)
         # 2. rho and its sector blocks
         rho = self.rdm()
+        if extra is not None:
+            one = dv.upload(np.ones(1), 0)
+            _lib.check(_lib.lib().zmpo_lincomb(0, dv.ptr(rho), dv.ptr(extra), m * m, 1,
+                                               dv.ptr(one), 1.0, 1.0, m * m, dv.stream()))
         pidx = np.concatenate([(r[:, None] * m + r[None, :]).reshape(-1) for r in rows])
         A = dv.empty(len(pidx), 0)
         gather(A, rho, dv.upload(pidx))

@@ -249,7 +249,13 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
             classes = dpt(dmrg.qnuml[isite], dmrg.qphys[isite])
         else:
             classes = dpt(dmrg.qphys[isite + 1], dmrg.qnumr[isite + 2])
-        if dd.run(classes, dmrg.thresh, dmrg.Dmax):
+        extra = None
+        noise = float(getattr(dmrg, 'noise', 0.0))
+        if noise >= 1e-10:
+            # the operator term of pRDMQt on the dense two-site vectors
+            extra = kernelQt.prdm_noise_device(dd.C, dd.neig, info, noise)
+            t0 = lap('noise', t0)
+        if dd.run(classes, dmrg.thresh, dmrg.Dmax, extra):
             site, srotR = dd.site_and_srotR(dmrg.qlst)
             t0 = lap('decimation', t0)
             if isite == actlst[-1]:
@@ -270,13 +276,12 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
 
 
 # Decimation and guess on the device (decim.py) unless ZMPO_DECIM=host;
-# the noise term and several symmetry sectors take the host path.
+# several symmetry sectors take the host path.
 DECIM = os.environ.get('ZMPO_DECIM', 'device')
 
 
 def _device_decimation(dmrg):
     return (DECIM == 'device' and len(dmrg.qsectors) == 1
-            and float(getattr(dmrg, 'noise', 0.0)) < 1e-10
             and dv.device().type == 'cuda')
 
 
@@ -302,6 +307,9 @@ TIERS = {1: (1,), 2: (2, 1)}
 TIER_TOL = {2: 4e-3, 1: 1e-6}
 # iterations of a BF16 phase (the last tier gets the solver's own budget)
 TIER_MAXCYCLE = {2: 12}
+# tiers whose phases also stop on the eigenvalue change (ZMPO_TIER_ECONV=0
+# turns it off: residual criterion only)
+TIER_ECONV = {1: os.environ.get('ZMPO_TIER_ECONV', '1') != '0', 2: False}
 TIER_HERM = {2: 2e-2, 1: 1e-5}
 
 
@@ -359,7 +367,11 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
         ph = EigenSolver()
         ph.maxcycle = TIER_MAXCYCLE.get(tier, solver.maxcycle) if tier != tiers[-1] \
             else solver.maxcycle
-        ph.crit_e = -1.0                  # residual criterion only
+        # FP32 phases also stop on the eigenvalue change (crit_e), as the
+        # FP64 Davidson does: the FP32 rounding moves theta = x.H'x by about
+        # |H'x - Hx| / sqrt(n), below crit_e for these sizes; BF16 phases
+        # stop on the residual only
+        ph.crit_e = dmrg.crit_e if TIER_ECONV.get(tier) else -1.0
         ph.ndim, ph.neig, ph.diag = n, 1, solver.diag
         ph.herm_tol = TIER_HERM[tier]
         ph.HVec = hvec_tier(tier)

@@ -974,9 +974,13 @@ class PRDMProgram:
             self.inter = [max(a, b) for a, b in zip(self.inter, offs)]
 
     def apply(self, mpo, env, cvals):
-        """Device buffer of every case's Gram blocks (case c at out_off[c])."""
-        t1 = dv.empty(self.inter[0], 0)
-        t2 = dv.empty(self.inter[1], 0)
+        """Device buffer of every case's Gram blocks (case c at out_off[c]);
+        the intermediates live in the shared scratch buffer of the sigma
+        and renormalisation launches."""
+        s1 = int(self.inter[0]) * 8
+        buf = dv.scratch('inter', s1 + int(self.inter[1]) * 8)
+        t1 = dv.typed(buf, 0, self.inter[0], 0)
+        t2 = dv.typed(buf, s1, self.inter[1], 0)
         out = dv.zeros(self.out_size, 0)
         E = env.buffer(0)
         W = mpo.buffer(self.site, 0)
@@ -1031,19 +1035,87 @@ def prdm_noise(cimat, info):
     status = info[4]
     env = flst[0][0] if status == 'L' else flst[0][1]
     keep = (0, 1) if status == 'L' else (1, 2)
+    tm = dmrg.__dict__.setdefault('timing', {})
+
+    def lap(key, t0):
+        t = time.time()
+        tm[key] = tm.get(key, 0.0) + t - t0
+        return t
     out = None
     idx = 0
     for key, neig in dmrg.qsectors.items():
+        t0 = time.time()
         qvec, idlst = prdm_template(dmrg, key, status)
         prog = PRDMProgram(dmrg.fhop, isite, qvec, env, status, np.array(key, dtype=np.float64),
                            getattr(dmrg, 'budget', DEFAULT_BUDGET))
+        t0 = lap('noise_plan', t0)
         for _ in range(neig):
             q = qvec.template()
             q.fromDense(np.asarray(cimat[idx]), idlst)
             idx += 1
             vals = prog.apply(dmrg.fhop, env, dv.upload(q.value, 0))
-            d = prog.dense(dv.download(vals), [idlst[k] for k in keep])
+            vals = dv.download(vals)
+            t0 = lap('noise_apply', t0)
+            d = prog.dense(vals, [idlst[k] for k in keep])
             out = d if out is None else out + d
+            t0 = lap('noise_dense', t0)
+    return out
+
+
+def prdm_noise_device(C, neig, info, scale=1.0):
+    """scale times the operator term of pRDMQt (prdm_noise) from the dense
+    two-site vectors on the device (C: float64, neig rows of ldim*16*rdim
+    in (l, n1, n2, r) order), as a device dense m x m matrix (m = ldim*4
+    for 'L', 4*rdim for 'R'): the C template's values are gathered from C
+    through its dense index map, the three launches of PRDMProgram run,
+    and each case's Gram blocks are scattered into the dense matrix and
+    summed -- no host copy of C or of the result.  One symmetry sector."""
+    dmrg, isite, flst = _info_parts(info)
+    status = info[4]
+    env = flst[0][0] if status == 'L' else flst[0][1]
+    keep = (0, 1) if status == 'L' else (1, 2)
+    if len(dmrg.qsectors) != 1:
+        raise NotImplementedError('prdm_noise_device: one symmetry sector')
+    tm = dmrg.__dict__.setdefault('timing', {})
+
+    def lap(key, t0):
+        dv.sync()
+        t = time.time()
+        tm[key] = tm.get(key, 0.0) + t - t0
+        return t
+    t0 = time.time()
+    key = next(iter(dmrg.qsectors))
+    qvec, idlst = prdm_template(dmrg, key, status)
+    prog = PRDMProgram(dmrg.fhop, isite, qvec, env, status, np.array(key, dtype=np.float64),
+                       getattr(dmrg, 'budget', DEFAULT_BUDGET))
+    kl = [idlst[k] for k in keep]
+    m = int(qvec.shape[keep[0]]) * int(qvec.shape[keep[1]])
+    ntot = int(np.prod([int(x) for x in qvec.shape]))
+    cmap = dv.upload(qvec.prjmap(idlst).astype(np.int64))
+    gmaps = {}
+    for c in prog.cases:
+        t3 = prog.plans[c][5].template()
+        if t3.size_allowed:
+            gmaps[c] = (t3.size_allowed, dv.upload(t3.prjmap([kl[0], kl[1], kl[0], kl[1]])
+                                                    .astype(np.int64)))
+    t0 = lap('noise_plan', t0)
+    out = dv.zeros(m * m, 0)
+    tmp = dv.empty(max([n for n, _ in gmaps.values()] + [1]), 0)
+    coef = dv.upload(np.ones(1), 0)
+    cv = dv.empty(qvec.size_allowed, 0)
+    for e in range(neig):
+        gather(cv, C[e * ntot:(e + 1) * ntot], cmap)
+        vals = prog.apply(dmrg.fhop, env, cv)
+        # out[map] += scale * blocks, case by case (a case's map has no
+        # repeated position; cases overlap, hence one case at a time)
+        for c, (n, gm) in gmaps.items():
+            o = prog.out_off[c]
+            gather(tmp[:n], out, gm)
+            _lib.check(_lib.lib().zmpo_lincomb(0, dv.ptr(tmp), dv.ptr(vals[o:o + n]), n, 1,
+                                               dv.ptr(coef), float(scale), 1.0, n, dv.stream()))
+            _lib.check(_lib.lib().zmpo_scatter(0, dv.ptr(out), 0, dv.ptr(tmp), dv.ptr(gm), n,
+                                               dv.stream()))
+    lap('noise_apply', t0)
     return out
 
 
