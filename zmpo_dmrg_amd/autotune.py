@@ -63,11 +63,18 @@ _online = {}
 
 
 def _wmedian(x, w):
+    """Weighted median of non-negative integer-valued x: the smallest value
+    whose cumulative weight (values in ascending order) reaches half the
+    total; by counting when the values are small, else by sorting."""
     if len(x) == 0 or w.sum() <= 0:
         return 1.0
-    o = np.argsort(x, kind='stable')
+    xi = x.astype(np.int64)
+    if xi.max() <= 1 << 20:
+        c = np.cumsum(np.bincount(xi, weights=w))
+        return float(np.searchsorted(c, 0.5 * c[-1]))
+    o = np.argsort(xi, kind='stable')
     c = np.cumsum(w[o])
-    return float(x[o][np.searchsorted(c, 0.5 * c[-1])])
+    return float(xi[o][np.searchsorted(c, 0.5 * c[-1])])
 
 
 def _bin(v):
@@ -75,7 +82,15 @@ def _bin(v):
 
 
 def signature(prog, role, dtype):
-    """(role, dtype, bins of M, N, summed K) of a grouped-GEMM program."""
+    """(role, dtype, bins of M, N, summed K) of a grouped-GEMM program (the
+    bins are computed once per program)."""
+    bins = getattr(prog, '_sig_bins', None)
+    if bins is None:
+        bins = prog._sig_bins = _size_bins(prog)
+    return (role, int(dtype)) + bins
+
+
+def _size_bins(prog):
     t, p = prog.tasks_host, prog.pairs_host
     cnt = (t['p_end'] - t['p_begin']).astype(np.int64)
     rows = np.repeat(np.arange(prog.ntasks), cnt)
@@ -85,8 +100,7 @@ def signature(prog, role, dtype):
     w = M * N * K
     ksum = np.bincount(rows, weights=K, minlength=prog.ntasks)
     wt = np.bincount(rows, weights=w, minlength=prog.ntasks)
-    return (role, int(dtype), _bin(_wmedian(M, w)), _bin(_wmedian(N, w)),
-            _bin(_wmedian(ksum, wt)))
+    return (_bin(_wmedian(M, w)), _bin(_wmedian(N, w)), _bin(_wmedian(ksum, wt)))
 
 
 def _load_table():

@@ -1296,6 +1296,7 @@ class ChainProgram:
         tfl = np.bincount(prow, weights=pfl, minlength=len(t)) if len(p) else np.zeros(len(t))
         self.instance_cases = [c for c, *_ in instances]
         templ = {}   # per case: unit records without the operator's bases, flops
+        bases = []
         for c, a0, b10, b20, c0 in instances:
             g = structures[c][0]
             if len(g) == 0:
@@ -1311,11 +1312,15 @@ class ChainProgram:
                 templ[c] = (u, int(tfl[self.groups_host['t1_begin'][gl]:
                                        self.groups_host['t2_end'][gl + len(g) - 1]].sum()))
             u0, fl = templ[c]
-            u = u0.copy()
-            u['a_base'], u['b1_base'], u['b2_base'], u['c_base'] = a0, b10, b20, c0
-            uparts.append(u)
+            uparts.append(u0)
+            bases.append((len(u0), a0, b10, b20, c0))
             self.flops += fl
+        # one copy of every operator's units, then its bases set per field
         self.units_host = np.concatenate(uparts) if uparts else np.zeros(0, _lib.CUNIT_DTYPE)
+        if bases:
+            bs = np.array(bases, dtype=np.int64)
+            for k, f in enumerate(('a_base', 'b1_base', 'b2_base', 'c_base')):
+                self.units_host[f] = np.repeat(bs[:, k + 1], bs[:, 0])
         self.nunits = len(self.units_host)
         # LDS classes, by the columns a group keeps in LDS: the intermediate,
         # plus the A tile when it is staged there (CHAIN_ATILE '1')
