@@ -279,6 +279,20 @@ class _CpuVecOps:
         rr = _np(r)
         rr[:] = rr / (np.abs(_np(diag) - e) + shift)
 
+    def copy(self, out, src):
+        _np(out)[:] = _np(src).reshape(-1)
+        return out
+
+    def scale(self, x, s):
+        xx = _np(x)
+        xx[:] = 0.0 if s == 0.0 else s * xx
+        return x
+
+    def axpy(self, out, a, x):
+        o = _np(out)
+        o[:] = o + a * _np(x).reshape(-1)
+        return out
+
 
 @contextlib.contextmanager
 def host_model():

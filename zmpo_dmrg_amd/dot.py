@@ -22,6 +22,7 @@ import time
 
 import numpy as np
 
+from . import _lib
 from . import device as dv
 from . import kernelQt
 from .pdvdson import DavidsonError, EigenSolver
@@ -158,9 +159,9 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
             elif dtype == 0:
                 prog.apply(dmrg.fhop, fL, fR, x, out, 0)
             else:
-                psi_lo.copy_(x)
+                _convert(psi_lo, dtype, x, 0)
                 prog.apply(dmrg.fhop, fL, fR, psi_lo, sig_lo, dtype)
-                out.copy_(sig_lo)
+                _convert(out, 0, sig_lo, kernelQt.sigma_dtype(dtype))
             dmrg.nsigma += 1
             dmrg.sigma_flops += prog.flops
 
@@ -180,9 +181,9 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
 
             def h(x, out):
                 pl, sl = lo_buffers(dt)
-                pl.copy_(x)
+                _convert(pl, dt, x, 0)
                 prog.apply(dmrg.fhop, fL, fR, pl, sl, dt)
-                out.copy_(sl)
+                _convert(out, 0, sl, kernelQt.sigma_dtype(dt))
                 dmrg.nsigma += 1
                 dmrg.nsigma_tier = getattr(dmrg, 'nsigma_tier', {})
                 dmrg.nsigma_tier[dt] = dmrg.nsigma_tier.get(dt, 0) + 1
@@ -279,6 +280,15 @@ def ci_solver(dmrg, isite, ncsite, actlst, flst, status, ifsym=True):
     return nmvps, np.array(eigs), sigs, dwts, qred, site, srotR
 
 
+def _convert(dst, dst_code, src, src_code):
+    """dst = src between storage types (0 FP64, 1 FP32, 2 BF16) on the
+    device (zmpo_convert; BF16 rounds to nearest even)."""
+    n = src.numel()
+    _lib.check(_lib.lib().zmpo_convert(dst_code, dv.ptr(dst), src_code, dv.ptr(src), n,
+                                       dv.stream()))
+    return dst
+
+
 # Decimation and guess on the device (decim.py) unless ZMPO_DECIM=host;
 # several symmetry sectors take the host path.
 DECIM = os.environ.get('ZMPO_DECIM', 'device')
@@ -344,19 +354,19 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
     the solve continues as a plain FP64 Davidson from the current vector.
     Returns (eigs, vec, nmvp)."""
     from .pdvdson import _VecOps
-    t = dv.torch()
     ops = _VecOps(n)
     if v0 is None:
         d = dv.download(solver.diag) + 1e-12 * np.arange(1, 1 + n) / float(n)
-        x = t.zeros(n, dtype=t.float64, device=dv.device())
-        x[int(np.argsort(d, kind='stable')[0])] = 1.0
+        e0 = np.zeros(n)
+        e0[int(np.argsort(d, kind='stable')[0])] = 1.0
+        x = dv.upload(e0, 0)
     elif isinstance(v0, np.ndarray):
         x = dv.upload(np.asarray(v0).reshape(-1), 0)
     else:
-        x = v0.reshape(-1).clone()
-    x.mul_(1.0 / ops.norm(x))
-    y = t.empty_like(x)
-    r = t.empty_like(x)
+        x = ops.copy(dv.empty(n, 0), v0.reshape(-1))
+    ops.scale(x, 1.0 / ops.norm(x))
+    y = dv.empty(n, 0)
+    r = dv.empty(n, 0)
     nmvp = 0
     niter = 0
     tiers = TIERS[dtype]
@@ -391,8 +401,8 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
             continue
         nmvp += nm
         niter += ph.niter
-        x.copy_(vec[0])
-        x.mul_(1.0 / ops.norm(x))
+        ops.copy(x, vec[0])
+        ops.scale(x, 1.0 / ops.norm(x))
         lap('tier%d' % tier, t0)
     # only the last tier runs again (inner solves, 'correction' mode):
     # release the others' intermediates before the FP64 ones are allocated
@@ -407,8 +417,8 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
         hvec64(x, y)
         nmvp += 1
         theta = float(ops.dots(x.view(1, -1), 1, y)[0])
-        r.copy_(y)
-        r.add_(x, alpha=-theta)
+        ops.copy(r, y)
+        ops.axpy(r, -theta, x)
         dmrg.last_residual64 = ops.norm(r)
         lap('rq64', t0)
         solver.niter = niter
@@ -429,8 +439,8 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
         hvec64(x, y)
         nmvp += 1
         theta = float(ops.dots(x.view(1, -1), 1, y)[0])
-        r.copy_(y)
-        r.add_(x, alpha=-theta)
+        ops.copy(r, y)
+        ops.axpy(r, -theta, x)
         rn = ops.norm(r)
         if rn <= dmrg.crit_vec or (theta_prev is not None
                                    and abs(theta - theta_prev) <= dmrg.crit_e):
@@ -454,8 +464,8 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
             return fallback()
         nmvp += nm
         niter += inner.niter
-        x.copy_(vec[0])
-        x.mul_(1.0 / ops.norm(x))
+        ops.copy(x, vec[0])
+        ops.scale(x, 1.0 / ops.norm(x))
     solver.niter = niter
     dmrg.nouter = getattr(dmrg, 'nouter', 0) + k + 1
     return np.array([theta]), x.view(1, -1), nmvp

@@ -23,6 +23,25 @@ class _VecOps:
         self.n = n
         self.work = dv.empty(_lib.lib().zmpo_dots_workspace(self.MAXD), 0)
         self.out = dv.empty(self.MAXD * self.MAXD, 0)
+        self.one = dv.upload(np.ones(1), 0)
+
+    def copy(self, out, src):
+        """out = src (FP64 vectors of length n)."""
+        _lib.check(_lib.lib().zmpo_lincomb(0, dv.ptr(out), dv.ptr(src), self.n, 1,
+                                           dv.ptr(self.one), 1.0, 0.0, self.n, dv.stream()))
+        return out
+
+    def scale(self, x, s):
+        """x *= s (s = 0 clears x)."""
+        _lib.check(_lib.lib().zmpo_lincomb(0, dv.ptr(x), None, self.n, 0, None, 1.0, float(s),
+                                           self.n, dv.stream()))
+        return x
+
+    def axpy(self, out, a, x):
+        """out += a x."""
+        _lib.check(_lib.lib().zmpo_lincomb(0, dv.ptr(out), dv.ptr(x), self.n, 1,
+                                           dv.ptr(self.one), float(a), 1.0, self.n, dv.stream()))
+        return out
 
     def _dots_into(self, X, nx, y, dst):
         _lib.check(_lib.lib().zmpo_dots(0, dv.ptr(X), X.stride(0), nx, dv.ptr(y), self.n,
@@ -53,10 +72,8 @@ class _VecOps:
     def lincomb(self, out, X, coef, alpha=1.0, beta=0.0):
         nx = len(coef)
         if nx == 0:
-            if beta == 0.0:
-                out.zero_()
-            elif beta != 1.0:
-                out.mul_(beta)
+            if beta != 1.0:
+                self.scale(out, beta)
             return out
         c = dv.upload(np.asarray(coef, dtype=np.float64), 0)
         _lib.check(_lib.lib().zmpo_lincomb(0, dv.ptr(out), dv.ptr(X), X.stride(0), nx, dv.ptr(c),
@@ -135,9 +152,9 @@ class EigenSolver:
             if rii <= self.crit_indp:
                 continue
             dst = V[out_start + nindp]
-            dst.copy_(R[i])
-            dst.mul_(1.0 / rii)
-            dst.mul_(1.0 / ops.norm(dst))
+            ops.copy(dst, R[i])
+            ops.scale(dst, 1.0 / rii)
+            ops.scale(dst, 1.0 / ops.norm(dst))
             nindp += 1
             for _ in range(5):
                 for j in rows[pos:]:
@@ -149,37 +166,44 @@ class EigenSolver:
         """Lowest neig eigenpairs.  w0 (neig = 1 only): the sigma of v0,
         already known (for instance in a higher precision than HVec): the
         first basis vector keeps it instead of calling HVec."""
-        t = dv.torch()
         n, neig = self.ndim, self.neig
         if neig > n:
             raise DavidsonError('neig > ndim')
         ops = _VecOps(n)
         maxv = self.nfac * neig + 2 * neig + 2
-        V = t.zeros((maxv, n), dtype=t.float64, device=dv.device())
-        W = t.zeros((maxv, n), dtype=t.float64, device=dv.device())
-        T = t.zeros((maxv, n), dtype=t.float64, device=dv.device())
-        R = t.zeros((neig, n), dtype=t.float64, device=dv.device())
+        # every row is written before it is read
+        V = dv.empty(maxv * n, 0).view(maxv, n)
+        W = dv.empty(maxv * n, 0).view(maxv, n)
+        T = dv.empty(maxv * n, 0).view(maxv, n)
+        R = dv.empty(neig * n, 0).view(neig, n)
         self.nmvp = 0
         if v0 is None:
             d = dv.download(self.diag) + 1e-12 * np.arange(1, 1 + n) / float(n)
             idx = np.argsort(d, kind='stable')[:neig]
-            for i, k in enumerate(idx):
-                V[i, int(k)] = 1.0
+            e = np.zeros((neig, n))
+            e[np.arange(neig), idx] = 1.0
+            ev = dv.upload(e.reshape(-1), 0).view(neig, n)
+            for i in range(neig):
+                ops.copy(V[i], ev[i])
         else:
-            V[:neig].copy_(v0 if not isinstance(v0, np.ndarray) else dv.upload(v0, 0).view(neig, n))
+            src = v0 if not isinstance(v0, np.ndarray) else dv.upload(v0, 0).view(neig, n)
+            for i in range(neig):
+                ops.copy(V[i], src.reshape(neig, n)[i])
         # genOrthoBas (pdvdson.py:335-345)
         if w0 is not None and neig != 1:
             raise DavidsonError('a given first sigma needs neig = 1')
         nrm0 = ops.norm(V[0])
-        V[0].mul_(1.0 / nrm0)
+        ops.scale(V[0], 1.0 / nrm0)
         if neig > 1:
-            R2 = V[1:neig].clone()
+            R2 = dv.empty((neig - 1) * n, 0).view(neig - 1, n)
+            for i in range(neig - 1):
+                ops.copy(R2[i], V[1 + i])
             nindp = self._ortho(ops, V, 1, R2, list(range(neig - 1)), 1)
             if nindp + 1 != neig:
                 raise DavidsonError('insufficient orthonormal basis')
         if w0 is not None:
-            W[0].copy_(w0.reshape(-1))
-            W[0].mul_(1.0 / nrm0)
+            ops.copy(W[0], w0.reshape(-1))
+            ops.scale(W[0], 1.0 / nrm0)
         else:
             for i in range(neig):
                 self.HVec(V[i], W[i])
@@ -216,10 +240,12 @@ class EigenSolver:
                 cm = np.vstack((vr, vr2)) if nindp else vr
                 for k in range(cm.shape[0]):
                     ops.lincomb(T[k], V, cm[k], 1.0, 0.0)
-                V[:cm.shape[0]].copy_(T[:cm.shape[0]])
+                for k in range(cm.shape[0]):
+                    ops.copy(V[k], T[k])
                 for k in range(cm.shape[0]):
                     ops.lincomb(T[k], W, cm[k], 1.0, 0.0)
-                W[:cm.shape[0]].copy_(T[:cm.shape[0]])
+                for k in range(cm.shape[0]):
+                    ops.copy(W[k], T[k])
                 nb = cm.shape[0]
                 vr = np.identity(nb)[:neig]
             for i in range(neig):
@@ -233,7 +259,7 @@ class EigenSolver:
                 self.HVec(V[nb + k], W[nb + k])
             self.nmvp += nindp
             nb += nindp
-        vec = t.zeros((neig, n), dtype=t.float64, device=dv.device())
+        vec = dv.empty(neig * n, 0).view(neig, n)
         for i in range(neig):
             ops.lincomb(vec[i], V, vr[i], 1.0, 0.0)
         return eigs, vec, self.nmvp
