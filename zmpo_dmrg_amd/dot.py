@@ -363,7 +363,7 @@ def _defect_correction(dmrg, solver, hvec_tier, hvec64, v0, n, dtype):
     elif isinstance(v0, np.ndarray):
         x = dv.upload(np.asarray(v0).reshape(-1), 0)
     else:
-        x = ops.copy(dv.empty(n, 0), v0.reshape(-1))
+        x = ops.copy(dv.empty(n, 0), v0.reshape(-1).contiguous())
     ops.scale(x, 1.0 / ops.norm(x))
     y = dv.empty(n, 0)
     r = dv.empty(n, 0)

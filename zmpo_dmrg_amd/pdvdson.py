@@ -186,9 +186,10 @@ class EigenSolver:
             for i in range(neig):
                 ops.copy(V[i], ev[i])
         else:
-            src = v0 if not isinstance(v0, np.ndarray) else dv.upload(v0, 0).view(neig, n)
+            src = v0 if not isinstance(v0, np.ndarray) else dv.upload(v0, 0)
+            src = src.reshape(neig, n).contiguous()
             for i in range(neig):
-                ops.copy(V[i], src.reshape(neig, n)[i])
+                ops.copy(V[i], src[i])
         # genOrthoBas (pdvdson.py:335-345)
         if w0 is not None and neig != 1:
             raise DavidsonError('a given first sigma needs neig = 1')
@@ -202,7 +203,7 @@ class EigenSolver:
             if nindp + 1 != neig:
                 raise DavidsonError('insufficient orthonormal basis')
         if w0 is not None:
-            ops.copy(W[0], w0.reshape(-1))
+            ops.copy(W[0], w0.reshape(-1).contiguous())
             ops.scale(W[0], 1.0 / nrm0)
         else:
             for i in range(neig):
