@@ -114,3 +114,52 @@ def test_host_budget_overflow_is_an_error():
         with pytest.raises(MemoryError):
             pd.MPODMRG(c.i1, c.i2, c.nelec, Dmax=8, tol=1e-9, mpo=c.mpo, env_budget=0,
                        host_budget=1)
+
+
+class _FakeEnv:
+    def __init__(self, n):
+        self.n, self.on = n, True
+
+    def nbytes(self):
+        return self.n
+
+    def resident(self):
+        return self.on
+
+    def spill(self):
+        self.on = False
+
+
+class _FakeDmrg:
+    def __init__(self, budget, host=None):
+        self.env_budget, self.host_budget, self.timing = budget, host, {}
+        self.envL = {j: _FakeEnv(10) for j in range(-1, 5)}
+        self.envR = {j: _FakeEnv(10) for j in range(4, 9)}
+
+
+def test_fit_envs_moves_the_farthest_read_first():
+    """'L' half-sweep at site 3: left environments go first, smallest bond
+    first (read last in the next 'R' half-sweep), then right ones from the
+    far end; the two the step reads (envL[2], envR[5]) never move."""
+    d = _FakeDmrg(budget=60)
+    pd.fit_envs(d, 3, 'L')
+    moved = [('L', j) for j, e in d.envL.items() if not e.on] + \
+        [('R', j) for j, e in d.envR.items() if not e.on]
+    # 11 stores of 10 bytes, budget 60: five move
+    assert moved == [('L', -1), ('L', 0), ('L', 1), ('L', 3), ('L', 4)]
+    assert d.envL[2].on and d.envR[5].on
+    # 'R' half-sweep at site 3 (reads envL[2], envR[5]): right ones from the
+    # far end first, then left ones from the smallest bond
+    d = _FakeDmrg(budget=60)
+    pd.fit_envs(d, 3, 'R')
+    moved = [('R', j) for j, e in d.envR.items() if not e.on] + \
+        [('L', j) for j, e in d.envL.items() if not e.on]
+    assert moved == [('R', 4), ('R', 6), ('R', 7), ('R', 8), ('L', -1)]
+    # no budget: nothing moves; a host budget too small raises first
+    d = _FakeDmrg(budget=None)
+    pd.fit_envs(d, 3, 'L')
+    assert all(e.on for e in list(d.envL.values()) + list(d.envR.values()))
+    d = _FakeDmrg(budget=60, host=25)
+    import pytest
+    with pytest.raises(MemoryError):
+        pd.fit_envs(d, 3, 'L')
