@@ -94,3 +94,51 @@ def test_sweeps_with_noise_reach_fci(gpu):
     d.noise = 0.0
     es = pdmrg.run(d, 2)
     assert abs(es[-1] - fci_energy(h, eri, K)[0]) < 1e-8
+
+
+def _device_maps_check(d, isite, status):
+    """The pipeline of kernelQt.prdm_noise_device, with its gather and
+    scatter-add written in numpy: C template values gathered from the
+    dense vector through qvec.prjmap, the PRDMProgram launches, each case's
+    Gram blocks added at t3.prjmap positions -- against the dense sum."""
+    flst = [[d.envL[isite - 1], d.envR[isite + 2]]]
+    ldim = int(flst[0][0].meta[0].shape[2])
+    rdim = int(flst[0][1].meta[0].shape[2])
+    d.dims = (ldim, 16, rdim, ldim * 16 * rdim)
+    setupQlst(d, isite)
+    setupQtmpCI(d, list(d.qsectors)[0])
+    q = d.qtmp.template()
+    q.value = np.random.default_rng(isite + 7).standard_normal(q.size_allowed)
+    C = q.toDenseTensor(d.idlstCI)
+    key = list(d.qsectors)[0]
+    qvec, idlst = kq.prdm_template(d, key, status)
+    cmap = qvec.prjmap(idlst)
+    ref = qvec.template()
+    ref.fromDense(C.reshape([int(x) for x in qvec.shape]), idlst)
+    assert np.array_equal(C.reshape(-1)[cmap], ref.value)
+    env = flst[0][0] if status == 'L' else flst[0][1]
+    prog = kq.PRDMProgram(d.fhop, isite, qvec, env, status, np.array(key), d.budget)
+    from zmpo_dmrg_amd import device as dv
+    vals = dv.download(prog.apply(d.fhop, env, dv.upload(C.reshape(-1)[cmap], 0)))
+    keep = (0, 1) if status == 'L' else (1, 2)
+    kl = [idlst[k] for k in keep]
+    m = int(qvec.shape[keep[0]]) * int(qvec.shape[keep[1]])
+    out = np.zeros(m * m)
+    for c in prog.cases:
+        t3 = prog.plans[c][5].template()
+        if t3.size_allowed:
+            gm = t3.prjmap([kl[0], kl[1], kl[0], kl[1]])
+            assert len(np.unique(gm)) == len(gm)        # no repeated position in a case
+            o = prog.out_off[c]
+            out[gm] += vals[o:o + t3.size_allowed]
+    want = _dense_noise(d, isite, C.reshape(qvec.shape), status)
+    assert np.abs(out.reshape(m, m) - want).max() < 1e-10 * max(1.0, np.abs(want).max())
+
+
+def test_device_noise_maps_host_model():
+    from kmodel import host_model
+    with host_model():
+        d = _state()
+        for isite in (1, 2):
+            for status in ('L', 'R'):
+                _device_maps_check(d, isite, status)
