@@ -314,6 +314,7 @@ def host_model():
 
     patch(dv, 'empty', lambda n, code=0: torch.zeros(int(max(n, 1)), dtype=torch.float64))
     patch(dv, 'zeros', lambda n, code=0: torch.zeros(int(max(n, 1)), dtype=torch.float64))
+    patch(dv, 'clear', lambda x, code=None: x.zero_())
     patch(dv, 'upload', upload)
     patch(dv, 'download', lambda x: np.array(_np(x)))
     patch(dv, 'device', lambda: torch.device('cpu'))

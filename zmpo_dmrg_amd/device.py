@@ -55,8 +55,22 @@ def empty(n, code=0):
 
 
 def zeros(n, code=0):
-    require()
-    return torch().zeros(int(max(n, 1)), dtype=tdtype(code), device=device())
+    """n zeros of storage type code on the launch stream."""
+    return clear(empty(n, code), code)
+
+
+def clear(x, code=None):
+    """x[:] = 0: FP64 / FP32 by the libzmpo lincomb kernel (no operands,
+    beta = 0) on the launch stream; other storage types by torch."""
+    if code is None:
+        code = {torch().float64: 0, torch().float32: 1}.get(x.dtype, -1)
+    if code in (0, 1):
+        from . import _lib
+        _lib.check(_lib.lib().zmpo_lincomb(code, ptr(x), None, 0, 0, None, 1.0, 0.0, x.numel(),
+                                           stream()))
+    else:
+        x.zero_()
+    return x
 
 
 ELSIZE = {0: 8, 1: 4, 2: 2, 3: 4}

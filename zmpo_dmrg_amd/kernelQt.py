@@ -520,7 +520,7 @@ class SigmaProgram:
                             stream=main.cuda_stream)
                 first = False
         if first:
-            sigma.zero_()
+            dv.clear(sigma)
         return sigma
 
 
