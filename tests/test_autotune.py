@@ -88,3 +88,22 @@ def test_kmajor_tile_needs_k_contiguous_operands():
     ok = pq.Program(t, p)
     ok.tiles[0] = ('shaped', 12)
     assert ok.kmajor_ok() and ok.launch_choice(0) == ('shaped', 12)
+
+
+def test_weighted_median_by_counting_equals_sorting():
+    """The counting weighted median of the signature equals the sorted
+    cumulative rule (the smallest value whose cumulative weight reaches half
+    the total) for small and large value ranges."""
+    rng = np.random.default_rng(0)
+
+    def by_sorting(x, w):
+        o = np.argsort(x, kind='stable')
+        c = np.cumsum(w[o])
+        return float(x[o][np.searchsorted(c, 0.5 * c[-1])])
+    for hi in (3, 100, 5000, 1 << 22):
+        for _ in range(200):
+            n = int(rng.integers(1, 300))
+            x = rng.integers(0, hi, n).astype(np.float64)
+            w = rng.integers(1, 1000, n).astype(np.float64)
+            assert at._wmedian(x, w) == by_sorting(x, w)
+    assert at._wmedian(np.zeros(0), np.zeros(0)) == 1.0
